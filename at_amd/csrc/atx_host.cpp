@@ -1,0 +1,788 @@
+// atx_host.cpp -- host runtime of the tracking engine: element attribute
+// store, lattice compiler (validation + host precompute into the device
+// format of atx_device.h), device memory, launch, and the C ABI of
+// include/atx.h.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/atx.h"
+#include "atx_device.h"
+
+extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, hipStream_t);
+
+namespace atx {
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// Constants of atintegrators/atconstants.h:15-31 (values, not code).
+const double kCbrt2 = 1.25992104989487316477;
+const double kDrift1 = 1.0 / (2.0 * (2.0 - kCbrt2));
+const double kDrift2 = 0.5 - (1.0 / (2.0 * (2.0 - kCbrt2)));
+const double kKick1 = 1.0 / (2.0 - kCbrt2);
+const double kKick2 = 1.0 - 2.0 / (2.0 - kCbrt2);
+const double kC0 = 2.99792458e8;
+const double kTwoPi = 2.0 * 3.14159265358979323846;
+const double kRe = 2.8179403205e-15;
+const double kE0GeV = 510.99895069e-6;
+const double kRadConst = 2.0 / 3.0 * kRe;
+
+struct Attr {
+  std::vector<double> v;
+  bool is_int = false;
+};
+
+struct PassInfo {
+  const char* name;
+  int kind;
+  bool mpole;
+};
+
+// Pass methods of the engine (names of atgpu/PassMethodFactory.cpp:33-46
+// restricted to the in-scope ones).
+const PassInfo kPasses[] = {
+    {"IdentityPass", K_IDENTITY, false},
+    {"DriftPass", K_DRIFT, false},
+    {"BndMPoleSymplectic4Pass", K_BND, true},
+    {"BndMPoleSymplectic4RadPass", K_BNDRAD, true},
+    {"StrMPoleSymplectic4Pass", K_STR, true},
+    {"StrMPoleSymplectic4RadPass", K_STRRAD, true},
+    {"RFCavityPass", K_RF, false},
+    {"ThinMPolePass", K_THIN, true},
+    {"CorrectorPass", K_CORR, false},
+    {"AperturePass", K_APERTURE, false},
+};
+
+struct HostElem {
+  std::string pass;
+  std::string fam;
+  int kind = 0;
+  std::map<std::string, Attr> attrs;
+
+  const Attr* get(const char* n) const {
+    auto it = attrs.find(n);
+    return it == attrs.end() ? nullptr : &it->second;
+  }
+};
+
+struct Compiled {
+  std::vector<DevElem> elems;
+  std::vector<double> pool;
+  double T0 = 0.0;
+};
+
+}  // namespace
+}  // namespace atx
+
+struct atx_lattice {
+  std::vector<atx::HostElem> elems;
+  // device copy of the compiled lattice, valid for (energy, rest, elempass)
+  bool dev_valid = false;
+  double dev_energy = -1.0, dev_rest = -1.0;
+  bool dev_elempass = false;
+  int dev_device = -1;
+  atx::DevElem* d_elems = nullptr;
+  double* d_pool = nullptr;
+  size_t d_elems_n = 0, d_pool_n = 0;
+  int64_t turn_counter = 0;
+  std::mutex mu;
+};
+
+namespace atx {
+namespace {
+
+// --- attribute access with the reference's required/optional semantics ----
+
+struct Reader {
+  const HostElem& e;
+  int idx;
+  int err = 0;
+
+  void missing(const char* n) {
+    if (!err)
+      err = fail(ATX_E_ATTRIBUTE, std::string("The required attribute ") + n +
+                                      " is missing (element #" + std::to_string(idx) + " " +
+                                      e.fam + ", " + e.pass + ")");
+  }
+  void bad(const std::string& m) {
+    if (!err) err = fail(ATX_E_VALUE, m + " (element #" + std::to_string(idx) + " " + e.fam + ")");
+  }
+  double dbl(const char* n) {
+    const Attr* a = e.get(n);
+    if (!a || a->v.empty()) {
+      missing(n);
+      return 0.0;
+    }
+    return a->v[0];
+  }
+  double opt_dbl(const char* n, double d) {
+    const Attr* a = e.get(n);
+    return (a && !a->v.empty()) ? a->v[0] : d;
+  }
+  long lng(const char* n) { return (long)dbl(n); }
+  long opt_lng(const char* n, long d) {
+    const Attr* a = e.get(n);
+    return (a && !a->v.empty()) ? (long)a->v[0] : d;
+  }
+  const std::vector<double>* arr(const char* n) {
+    const Attr* a = e.get(n);
+    if (!a) {
+      missing(n);
+      return nullptr;
+    }
+    return &a->v;
+  }
+  const std::vector<double>* opt_arr(const char* n, size_t need) {
+    const Attr* a = e.get(n);
+    if (!a) return nullptr;
+    if (need && a->v.size() < need) {
+      bad(std::string("attribute ") + n + " has " + std::to_string(a->v.size()) +
+          " values, expected " + std::to_string(need));
+      return nullptr;
+    }
+    return &a->v;
+  }
+};
+
+int32_t push(std::vector<double>& pool, const double* v, size_t n) {
+  int32_t off = (int32_t)pool.size();
+  pool.insert(pool.end(), v, v + n);
+  return off;
+}
+
+// Common optional misalignment and aperture attributes.
+void transforms(Reader& rd, DevElem& h, std::vector<double>& pool) {
+  if (auto t = rd.opt_arr("T1", 6)) { h.flags |= F_T1; h.t1 = push(pool, t->data(), 6); }
+  if (auto t = rd.opt_arr("R1", 36)) { h.flags |= F_R1; h.r1 = push(pool, t->data(), 36); }
+  if (auto t = rd.opt_arr("R2", 36)) { h.flags |= F_R2; h.r2 = push(pool, t->data(), 36); }
+  if (auto t = rd.opt_arr("T2", 6)) { h.flags |= F_T2; h.t2 = push(pool, t->data(), 6); }
+  if (auto t = rd.opt_arr("RApertures", 4)) { h.flags |= F_RAP; h.rap = push(pool, t->data(), 4); }
+  if (auto t = rd.opt_arr("EApertures", 2)) { h.flags |= F_EAP; h.eap = push(pool, t->data(), 2); }
+}
+
+// elegant linear fringe blocks (quadfringe.c:52-93) for one part.
+void elegant_block(double k1, const double* fi, int part, bool swap, double out[8]) {
+  const double k1sq = k1 * k1;
+  double j1x, j2x, j3x, j1y;
+  if (part == 1) {
+    j1x = 1.0 * (k1 * fi[1] - 2 * k1sq * fi[3] / 3. - k1sq * fi[0] * fi[2] / 2);
+    j2x = 1.0 * k1 * fi[2];
+    j3x = 1.0 * k1sq * (fi[2] + fi[4] + fi[0] * fi[1]);
+    j1y = 1.0 * (-k1 * fi[1] - 2 * k1sq * fi[3] / 3. - k1sq * fi[0] * fi[2] / 2);
+  } else {
+    j1x = 1.0 * (k1 * fi[1] + k1sq * fi[0] * fi[2] / 2);
+    j2x = 1.0 * k1 * fi[2];
+    j3x = 1.0 * k1sq * (fi[4] - fi[0] * fi[1]);
+    j1y = 1.0 * (-k1 * fi[1] + k1sq * fi[0] * fi[2] / 2);
+  }
+  const double j2y = -j2x, j3y = j3x;
+  const double ex = std::exp(j1x), ey = std::exp(j1y);
+  double r00 = ex, r01 = j2x / ex, r10 = ex * j3x, r11 = (1 + j2x * j3x) / ex;
+  double r22 = ey, r23 = j2y / ey, r32 = ey * j3y, r33 = (1 + j2y * j3y) / ey;
+  if (swap) {
+    std::swap(r00, r11);
+    std::swap(r22, r33);
+  }
+  out[0] = r00; out[1] = r01; out[2] = r10; out[3] = r11;
+  out[4] = r22; out[5] = r23; out[6] = r32; out[7] = r33;
+}
+
+// Edge-focusing mode and constants for one side of a bend
+// (atphyslib.c:15-79 non-radiating, diff_bend_fringe.c:2-53 radiating).
+int32_t edge_mode(bool rad, int method, double irho, double angle, double fint, double gap,
+                  double* fx, double* c0, double* corr) {
+  double cr;
+  if ((fint == 0.0) || (gap == 0.0) || (method == 0)) {
+    cr = 0.0;
+  } else {
+    const double se = std::sin(angle), ce = std::cos(angle);
+    cr = rad ? irho * gap * fint * (1.0 + se * se) / ce : irho * gap * fint * (1 + se * se) / ce;
+  }
+  *fx = irho * std::tan(angle);
+  *c0 = *fx;
+  *corr = cr;
+  if (method == 3) return rad ? E_RTAN3 : E_TAN3;
+  if (method == 2) {
+    if (cr == 0.0) return rad ? E_MULPN : E_DIVP;
+    return rad ? E_RTAN2 : E_TAN2;
+  }
+  if (cr == 0.0) return E_CONST;
+  return rad ? E_RTAN1 : E_TAN1;
+}
+
+int compile_elem(const HostElem& e, int idx, double energy, double rest, double T0,
+                 Compiled& out) {
+  Reader rd{e, idx};
+  DevElem h;
+  std::memset(&h, 0, sizeof(h));
+  h.kind = e.kind;
+  std::vector<double>& pool = out.pool;
+  double sc[S_NSCALAR];
+  for (double& v : sc) v = 0.0;
+
+  switch (e.kind) {
+    case K_IDENTITY:
+      transforms(rd, h, pool);
+      break;
+    case K_DRIFT:
+      sc[S_D_LEN] = rd.dbl("Length");
+      transforms(rd, h, pool);
+      break;
+    case K_APERTURE: {
+      auto lim = rd.arr("Limits");
+      if (lim && lim->size() < 4) rd.bad("Limits must have 4 values");
+      if (!rd.err) h.rap = push(pool, lim->data(), 4);
+    } break;
+    case K_CORR: {
+      const double len = rd.dbl("Length");
+      auto ka = rd.arr("KickAngle");
+      if (ka && ka->size() < 2) rd.bad("KickAngle must have 2 values");
+      const double s = rd.opt_dbl("FieldScaling", 1.0);
+      transforms(rd, h, pool);
+      if (rd.err) break;
+      sc[S_C_XK] = (*ka)[0];
+      sc[S_C_YK] = (*ka)[1];
+      sc[S_C_LEN] = len;
+      sc[S_SCALE] = s;
+      sc[S_INVSCALE] = 1.0 / s;
+      if (s != 1.0) h.flags |= F_SCALE;
+      if (len != 0.0) h.flags |= F_THICK;
+    } break;
+    case K_RF: {
+      const double len = rd.dbl("Length");
+      const double volt = rd.dbl("Voltage");
+      const double freq = rd.dbl("Frequency");
+      const double lag = rd.opt_dbl("TimeLag", 0.0);
+      const double philag = rd.opt_dbl("PhaseLag", 0.0);
+      if (rd.err) break;
+      if (!(energy > 0.0)) {
+        rd.err = fail(ATX_E_VALUE, "Energy must be positive. Check lattice or set the track() keyword argument.");
+        break;
+      }
+      const double harm = std::round(freq * T0);  // RFCavityPass.c:59
+      const double nv = volt / energy;            // RFCavityPass.c:65
+      sc[S_RF_HALF] = len / 2;
+      sc[S_RF_NV] = nv;
+      sc[S_RF_TWOPIF] = kTwoPi * freq;
+      sc[S_RF_LAG] = lag;
+      sc[S_RF_PHILAG] = philag;
+      sc[S_RF_HFT0] = harm / freq - T0;
+      if (len != 0.0) h.flags |= F_THICK;
+      if (nv != 0.0) h.flags |= F_RFKICK;
+    } break;
+    default: {  // multipoles
+      const bool thin = (e.kind == K_THIN);
+      const bool bend = (e.kind == K_BND || e.kind == K_BNDRAD);
+      const bool rad = (e.kind == K_STRRAD || e.kind == K_BNDRAD);
+      const double len = thin ? 0.0 : rd.dbl("Length");
+      auto pa = rd.arr("PolynomA");
+      auto pb = rd.arr("PolynomB");
+      const long order = rd.lng("MaxOrder");
+      const long nslice = thin ? 0 : rd.lng("NumIntSteps");
+      double bang = 0.0, ein = 0.0, eout = 0.0;
+      if (bend) {
+        bang = rd.dbl("BendingAngle");
+        ein = rd.dbl("EntranceAngle");
+        eout = rd.dbl("ExitAngle");
+      }
+      if (rd.err) break;
+      if (order < 0 || order > 30) {
+        rd.bad("MaxOrder out of range");
+        break;
+      }
+      const size_t need = (size_t)order + 1 > (thin ? 1u : 2u) ? (size_t)order + 1 : (thin ? 1u : 2u);
+      if (pa->size() < (size_t)order + 1 || pb->size() < need) {
+        rd.bad("PolynomA/PolynomB shorter than MaxOrder+1");
+        break;
+      }
+      const double s = rd.opt_dbl("FieldScaling", 1.0);
+      auto ka = rd.opt_arr("KickAngle", 2);
+      transforms(rd, h, pool);
+      if (rd.err) break;
+      h.order = (int32_t)order;
+      h.nslice = (int32_t)nslice;
+      std::vector<double> a(pa->begin(), pa->begin() + order + 1);
+      std::vector<double> b(pb->begin(), pb->begin() + order + 1);
+      if (ka) {
+        if (thin) {  // ThinMPolePass.c:38-41
+          b[0] -= (*ka)[0];
+          a[0] += (*ka)[1];
+        } else {  // StrMPoleSymplectic4Pass.c:49-52
+          b[0] -= std::sin((*ka)[0]) / len;
+          a[0] += std::sin((*ka)[1]) / len;
+        }
+      }
+      h.poly = push(pool, a.data(), a.size());
+      push(pool, b.data(), b.size());
+      sc[S_SCALE] = s;
+      sc[S_INVSCALE] = 1.0 / s;
+      if (s != 1.0) h.flags |= F_SCALE;
+      if (thin) {
+        if (const Attr* ba = e.get("BendingAngle")) {
+          if (ba->v.size() >= 1) sc[S_BAX] = ba->v[0];
+          if (ba->v.size() >= 2) sc[S_BAY] = ba->v[1];
+        }
+        break;
+      }
+      const double sl = len / (double)nslice;
+      sc[S_L1] = sl * kDrift1;
+      sc[S_L2] = sl * kDrift2;
+      sc[S_K1] = sl * kKick1;
+      sc[S_K2] = sl * kKick2;
+      const double b2 = (*pb)[1];
+      sc[S_B2] = b2;
+      const long fqi = rd.opt_lng("FringeQuadEntrance", 0);
+      const long fqo = rd.opt_lng("FringeQuadExit", 0);
+      auto fim = rd.opt_arr("fringeIntM0", 5);
+      auto fip = rd.opt_arr("fringeIntP0", 5);
+      if (rd.err) break;
+      const int qin = (fqi && b2 != 0) ? ((fqi == 2 && fim && fip) ? Q_ELEGANT : Q_LEE) : Q_NONE;
+      const int qout = (fqo && b2 != 0) ? ((fqo == 2 && fim && fip) ? Q_ELEGANT : Q_LEE) : Q_NONE;
+      h.fringe = qin | (qout << 8);
+      if (qin == Q_ELEGANT || qout == Q_ELEGANT) {
+        double m[32];
+        elegant_block(b2, fip->data(), 2, true, m);        // entrance, first matrix
+        elegant_block(b2, fim->data(), 1, true, m + 8);    // entrance, second matrix
+        elegant_block(b2, fim->data(), 1, false, m + 16);  // exit, first matrix
+        elegant_block(b2, fip->data(), 2, false, m + 24);  // exit, second matrix
+        h.fq = push(pool, m, 32);
+      }
+      if (bend) {
+        const double irho = bang / len;
+        sc[S_IRHO] = irho;
+        const int mi = (int)rd.opt_lng("FringeBendEntrance", 1);
+        const int mo = (int)rd.opt_lng("FringeBendExit", 1);
+        const double gap = rd.opt_dbl("FullGap", 0.0);
+        const double f1 = rd.opt_dbl("FringeInt1", 0.0);
+        const double f2 = rd.opt_dbl("FringeInt2", 0.0);
+        h.edge_in = edge_mode(rad, mi, irho, ein, f1, gap, &sc[S_FX_IN], &sc[S_FY_IN], &sc[S_CORR_IN]);
+        sc[S_EDGE_IN] = ein;
+        h.edge_out = edge_mode(rad, mo, irho, eout, f2, gap, &sc[S_FX_OUT], &sc[S_FY_OUT], &sc[S_CORR_OUT]);
+        sc[S_EDGE_OUT] = eout;
+      }
+      if (rad) {
+        if (!(energy > 0.0)) {
+          rd.err = fail(ATX_E_VALUE, "Energy must be positive. Check lattice or set the track() keyword argument.");
+          break;
+        }
+        const double gamma = (rest == 0.0) ? 1.0E-9 * energy / kE0GeV : energy / rest;
+        sc[S_RADC] = kRadConst * std::pow(gamma, 3);
+      }
+    } break;
+  }
+  if (rd.err) return rd.err;
+  h.sc = push(pool, sc, S_NSCALAR);
+  out.elems.push_back(h);
+  return ATX_OK;
+}
+
+int compile_lattice(const atx_lattice* lat, double energy, double rest, bool elempass,
+                    Compiled& out) {
+  double ring_length = 0.0;
+  for (const auto& e : lat->elems) {
+    const Attr* a = e.get("Length");
+    if (a && !a->v.empty()) ring_length += a->v[0];
+  }
+  out.T0 = elempass ? 0.0 : ring_length / kC0;  // at.c:540-551, at.c:728-729
+  out.elems.clear();
+  out.pool.clear();
+  for (size_t i = 0; i < lat->elems.size(); i++) {
+    int rc = compile_elem(lat->elems[i], (int)i, energy, rest, out.T0, out);
+    if (rc) return rc;
+  }
+  if (out.pool.empty()) out.pool.push_back(0.0);
+  return ATX_OK;
+}
+
+// Refpt copy ranks following at.c:618-651 (the output pointer advances
+// once per copy, so a copy's slot is its rank in the copy sequence).
+int turn_slots(const uint32_t* refpts, int nref, int start, int end, int nelem, int32_t* slot) {
+  for (int e = 0; e <= nelem; e++) slot[e] = -1;
+  int idx = 0;
+  uint32_t next = (idx < nref) ? refpts[idx++] : (uint32_t)INT_MAX;
+  int k = 0;
+  for (int e = start; e < end; e++) {
+    if ((uint32_t)e == next) {
+      slot[e] = k++;
+      next = (idx < nref) ? refpts[idx++] : (uint32_t)INT_MAX;
+    }
+  }
+  if ((uint32_t)nelem == next) slot[nelem] = k++;
+  return k;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ATX_E_RUNTIME, std::string(#x) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool elempass) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (lat->dev_valid && lat->dev_energy == energy && lat->dev_rest == rest &&
+      lat->dev_elempass == elempass && lat->dev_device == dev)
+    return ATX_OK;
+  Compiled c;
+  int rc = compile_lattice(lat, energy, rest, elempass, c);
+  if (rc) return rc;
+  if (lat->d_elems) (void)hipFree(lat->d_elems);
+  if (lat->d_pool) (void)hipFree(lat->d_pool);
+  lat->d_elems = nullptr;
+  lat->d_pool = nullptr;
+  lat->dev_valid = false;
+  const size_t ne = c.elems.empty() ? 1 : c.elems.size();
+  HIPCHK(hipMalloc(&lat->d_elems, ne * sizeof(DevElem)));
+  HIPCHK(hipMalloc(&lat->d_pool, c.pool.size() * sizeof(double)));
+  if (!c.elems.empty())
+    HIPCHK(hipMemcpy(lat->d_elems, c.elems.data(), c.elems.size() * sizeof(DevElem), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(lat->d_pool, c.pool.data(), c.pool.size() * sizeof(double), hipMemcpyHostToDevice));
+  lat->d_elems_n = c.elems.size();
+  lat->d_pool_n = c.pool.size();
+  lat->dev_valid = true;
+  lat->dev_energy = energy;
+  lat->dev_rest = rest;
+  lat->dev_elempass = elempass;
+  lat->dev_device = dev;
+  return ATX_OK;
+}
+
+constexpr int kBlock = 256;
+
+// Device-side core of atx_track: all pointers are device pointers.
+int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
+                 int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
+                 double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
+                 int losses, bool elempass, hipStream_t stream, bool strict) {
+  const int nelem = (int)lat->elems.size();
+  std::vector<int32_t> slots(2 * (size_t)(nelem + 1));
+  const int n_first = turn_slots(refpts, (int)nref, start, end, nelem, slots.data());
+  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, slots.data() + nelem + 1);
+  if ((n_first > 0 || n_std > 0) && !d_rout) return fail(ATX_E_VALUE, "r_out is NULL but refpts request copies");
+  if (losses && (!d_lf || !d_lt || !d_le || !d_lc)) return fail(ATX_E_VALUE, "loss arrays are NULL");
+  if (np == 0 || nturns <= 0) return ATX_OK;
+
+  Instance inst;
+  std::memset(&inst, 0, sizeof(inst));
+  inst.elems = lat->d_elems;
+  inst.pool = lat->d_pool;
+  inst.r = d_r;
+  inst.rout = d_rout;
+  inst.lost_flag = d_lf;
+  inst.lost_turn = d_lt;
+  inst.lost_elem = d_le;
+  inst.lost_coord = d_lc;
+  inst.pstride = pstride;
+  inst.cstride = cstride;
+  inst.nelem = nelem;
+  inst.npart = (int32_t)np;
+  inst.nturns = nturns;
+  inst.turn0 = turn0;
+  inst.start_elem = start;
+  inst.end_elem = end;
+  inst.n_first = n_first;
+  inst.n_std = n_std;
+  inst.block_begin = 0;
+  inst.losses = losses ? 1 : 0;
+  inst.elempass = elempass ? 1 : 0;
+
+  // One stream-ordered scratch block: [Instance][slots]
+  const size_t bytes = sizeof(Instance) + slots.size() * sizeof(int32_t);
+  std::vector<char> host(bytes);
+  char* d_scratch = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d_scratch, bytes, stream));
+  inst.slots = (const int32_t*)(d_scratch + sizeof(Instance));
+  std::memcpy(host.data(), &inst, sizeof(Instance));
+  std::memcpy(host.data() + sizeof(Instance), slots.data(), slots.size() * sizeof(int32_t));
+  HIPCHK(hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream));
+  const int nblocks = (int)((np + kBlock - 1) / kBlock);
+  hipError_t le = strict ? atx_launch_track_strict((const Instance*)d_scratch, 1, nblocks, kBlock, stream)
+                         : atx_launch_track_fast((const Instance*)d_scratch, 1, nblocks, kBlock, stream);
+  hipError_t fe = hipFreeAsync(d_scratch, stream);
+  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  // hipMemcpyAsync from pageable memory returns once the source is staged,
+  // so `host` may go out of scope here.
+  return ATX_OK;
+}
+
+int check_partial(int nelem, int nturns, int& start, int& end) {
+  // at.c:514-536
+  if (end == 0) end = nelem;
+  if (nelem != 0) {
+    if (start < 0 || start >= nelem)
+      return fail(ATX_E_VALUE, "start_elem out of range, " + std::to_string(start) + " not in [0.." +
+                                   std::to_string(nelem - 1) + "]");
+    if (end < 0 || end > nelem)
+      return fail(ATX_E_VALUE, "end_elem out of range, " + std::to_string(end) + " not in [0.." +
+                                   std::to_string(nelem) + "]");
+    if (start > end) return fail(ATX_E_VALUE, "end_elem must be greater that start_elem");
+    if (end != nelem && nturns != 1)
+      return fail(ATX_E_VALUE, "partial turn tracking not ending at the end of the ring requires nturns=1");
+  } else {
+    start = 0;
+    end = 0;
+  }
+  return ATX_OK;
+}
+
+}  // namespace
+}  // namespace atx
+
+using namespace atx;
+
+extern "C" {
+
+const char* atx_last_error(void) { return g_err.c_str(); }
+
+int atx_lattice_create(atx_lattice** out) {
+  if (!out) return fail(ATX_E_VALUE, "NULL handle pointer");
+  *out = new atx_lattice();
+  return ATX_OK;
+}
+
+int atx_lattice_destroy(atx_lattice* lat) {
+  if (!lat) return ATX_OK;
+  if (lat->d_elems) (void)hipFree(lat->d_elems);
+  if (lat->d_pool) (void)hipFree(lat->d_pool);
+  delete lat;
+  return ATX_OK;
+}
+
+int atx_lattice_add_element(atx_lattice* lat, const char* pass_method, const char* fam_name) {
+  if (!lat || !pass_method) return fail(ATX_E_VALUE, "NULL argument");
+  for (const auto& p : kPasses) {
+    if (std::strcmp(p.name, pass_method) == 0) {
+      std::lock_guard<std::mutex> g(lat->mu);
+      HostElem e;
+      e.pass = pass_method;
+      e.fam = fam_name ? fam_name : "";
+      e.kind = p.kind;
+      lat->elems.push_back(std::move(e));
+      lat->dev_valid = false;
+      return (int)ATX_OK;
+    }
+  }
+  return fail(ATX_E_RUNTIME, std::string("Not implemented PassMethod: ") + pass_method +
+                                 " in element #" + std::to_string(lat->elems.size()) + " (" +
+                                 (fam_name ? fam_name : "") + ")");
+}
+
+static int set_attr(atx_lattice* lat, const char* name, const double* v, int64_t n, bool is_int) {
+  if (!lat || !name) return fail(ATX_E_VALUE, "NULL argument");
+  if (lat->elems.empty()) return fail(ATX_E_VALUE, "no element to set an attribute on");
+  if (n < 0 || (n > 0 && !v)) return fail(ATX_E_VALUE, "bad array");
+  std::lock_guard<std::mutex> g(lat->mu);
+  Attr a;
+  a.v.assign(v, v + n);
+  a.is_int = is_int;
+  lat->elems.back().attrs[name] = std::move(a);
+  lat->dev_valid = false;
+  return ATX_OK;
+}
+
+int atx_element_set_double(atx_lattice* lat, const char* name, double value) {
+  return set_attr(lat, name, &value, 1, false);
+}
+
+int atx_element_set_int(atx_lattice* lat, const char* name, int64_t value) {
+  double v = (double)value;
+  return set_attr(lat, name, &v, 1, true);
+}
+
+int atx_element_set_array(atx_lattice* lat, const char* name, const double* data, int64_t n) {
+  return set_attr(lat, name, data, n, false);
+}
+
+int atx_lattice_num_elements(const atx_lattice* lat) { return lat ? (int)lat->elems.size() : 0; }
+
+int atx_lattice_check(atx_lattice* lat, double energy, double rest_energy) {
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  Compiled c;
+  return compile_lattice(lat, energy, rest_energy, false, c);
+}
+
+int atx_lattice_turn_counter(const atx_lattice* lat, int64_t* turn) {
+  if (!lat || !turn) return fail(ATX_E_VALUE, "NULL argument");
+  *turn = lat->turn_counter;
+  return ATX_OK;
+}
+
+int atx_track(atx_lattice* lat, double* r_in, uint32_t np, int32_t nturns, const uint32_t* refpts,
+              uint32_t nref, int32_t turn, double energy, double rest_energy, double charge,
+              int32_t losses, int32_t start_elem, int32_t end_elem, double* r_out,
+              uint8_t* lost_flag, uint32_t* lost_turn, uint32_t* lost_elem, double* lost_coord,
+              void* stream_, uint32_t flags) {
+  (void)charge;
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  if (np > 0 && !r_in) return fail(ATX_E_VALUE, "r_in is NULL");
+  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  if (nref > 0 && !refpts) return fail(ATX_E_VALUE, "refpts is NULL");
+  for (uint32_t i = 1; i < nref; i++)
+    if (refpts[i] < refpts[i - 1]) return fail(ATX_E_VALUE, "refpts must be sorted");
+  std::lock_guard<std::mutex> g(lat->mu);
+  const int nelem = (int)lat->elems.size();
+  int start = start_elem, end = end_elem;
+  int rc = check_partial(nelem, nturns, start, end);
+  if (rc) return rc;
+  rc = ensure_device_lattice(lat, energy, rest_energy, false);
+  if (rc) return rc;
+  const int turn0 = (flags & ATX_KEEP_COUNTER) ? (int)lat->turn_counter : turn;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool strict = (flags & ATX_STRICT) != 0;
+
+  // the (6,N,R,T) copy count the reference produces
+  std::vector<int32_t> tmp(nelem + 1);
+  const int n_first = nturns > 0 ? turn_slots(refpts, (int)nref, start, end, nelem, tmp.data()) : 0;
+  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, tmp.data());
+  (void)n_first;
+  (void)n_std;
+
+  if (flags & ATX_DEVICE_POINTERS) {
+    const bool soa = (flags & ATX_SOA) != 0;
+    rc = track_device(lat, r_in, soa ? 1 : 6, soa ? (int64_t)np : 1, np, nturns, refpts, nref, turn0,
+                      start, end, r_out, lost_flag, lost_turn, lost_elem, lost_coord, losses, false,
+                      stream, strict);
+    if (rc) return rc;
+  } else {
+    // host buffers: stage through device memory
+    const size_t nr = 6 * (size_t)np;
+    const size_t nout = nr * (size_t)nref * (size_t)nturns;
+    double *d_r = nullptr, *d_out = nullptr, *d_lc = nullptr;
+    uint8_t* d_lf = nullptr;
+    uint32_t *d_lt = nullptr, *d_le = nullptr;
+    auto cleanup = [&]() {
+      if (d_r) (void)hipFree(d_r);
+      if (d_out) (void)hipFree(d_out);
+      if (d_lc) (void)hipFree(d_lc);
+      if (d_lf) (void)hipFree(d_lf);
+      if (d_lt) (void)hipFree(d_lt);
+      if (d_le) (void)hipFree(d_le);
+    };
+#define HIPCHK2(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) {                                                                 \
+      cleanup();                                                                            \
+      return fail(ATX_E_RUNTIME, std::string(#x) + ": " + hipGetErrorString(e_));           \
+    }                                                                                       \
+  } while (0)
+    if (np > 0) {
+      HIPCHK2(hipMalloc(&d_r, nr * sizeof(double)));
+      HIPCHK2(hipMemcpy(d_r, r_in, nr * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (nout > 0 && r_out) {
+      HIPCHK2(hipMalloc(&d_out, nout * sizeof(double)));
+      // slots the reference never writes keep the caller's content
+      HIPCHK2(hipMemcpy(d_out, r_out, nout * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (losses && np > 0) {
+      HIPCHK2(hipMalloc(&d_lc, nr * sizeof(double)));
+      HIPCHK2(hipMalloc(&d_lf, np));
+      HIPCHK2(hipMalloc(&d_lt, np * sizeof(uint32_t)));
+      HIPCHK2(hipMalloc(&d_le, np * sizeof(uint32_t)));
+    }
+    rc = track_device(lat, d_r, 6, 1, np, nturns, refpts, nref, turn0, start, end, d_out, d_lf, d_lt,
+                      d_le, d_lc, losses, false, stream, strict);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    HIPCHK2(hipStreamSynchronize(stream));
+    if (np > 0) HIPCHK2(hipMemcpy(r_in, d_r, nr * sizeof(double), hipMemcpyDeviceToHost));
+    if (d_out) HIPCHK2(hipMemcpy(r_out, d_out, nout * sizeof(double), hipMemcpyDeviceToHost));
+    if (losses && np > 0) {
+      HIPCHK2(hipMemcpy(lost_coord, d_lc, nr * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK2(hipMemcpy(lost_flag, d_lf, np, hipMemcpyDeviceToHost));
+      HIPCHK2(hipMemcpy(lost_turn, d_lt, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIPCHK2(hipMemcpy(lost_elem, d_le, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    cleanup();
+#undef HIPCHK2
+  }
+  lat->turn_counter = (int64_t)turn0 + nturns;  // at.c:652,656
+  return ATX_OK;
+}
+
+int atx_elempass(atx_lattice* lat, int32_t index, double* r_in, uint32_t np, double energy,
+                 double rest_energy, void* stream_, uint32_t flags) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  std::lock_guard<std::mutex> g(lat->mu);
+  const int nelem = (int)lat->elems.size();
+  if (index < 0 || index >= nelem) return fail(ATX_E_VALUE, "element index out of range");
+  if (np == 0) return ATX_OK;
+  if (!r_in) return fail(ATX_E_VALUE, "r_in is NULL");
+  int rc = ensure_device_lattice(lat, energy, rest_energy, true);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool strict = (flags & ATX_STRICT) != 0;
+  if (flags & ATX_DEVICE_POINTERS)
+    return track_device(lat, r_in, 6, 1, np, 1, nullptr, 0, 0, index, index + 1, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, 0, true, stream, strict);
+  double* d_r = nullptr;
+  const size_t nr = 6 * (size_t)np;
+  HIPCHK(hipMalloc(&d_r, nr * sizeof(double)));
+  hipError_t e = hipMemcpy(d_r, r_in, nr * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    rc = track_device(lat, d_r, 6, 1, np, 1, nullptr, 0, 0, index, index + 1, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, 0, true, stream, strict);
+    if (!rc) e = hipMemcpy(r_in, d_r, nr * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(d_r);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  return ATX_OK;
+}
+
+int atx_device_count(int* count) {
+  if (!count) return fail(ATX_E_VALUE, "NULL argument");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(ATX_E_NODEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  *count = n;
+  return ATX_OK;
+}
+
+int atx_device_info(int device, char* name, int name_len, char* arch, int arch_len, int* cus,
+                    char* platform, int platform_len) {
+  hipDeviceProp_t p;
+  hipError_t e = hipGetDeviceProperties(&p, device);
+  if (e != hipSuccess) return fail(ATX_E_NODEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+  int rv = 0;
+  (void)hipRuntimeGetVersion(&rv);
+  if (name && name_len > 0) std::snprintf(name, name_len, "%s", p.name);
+  if (arch && arch_len > 0) std::snprintf(arch, arch_len, "%s", p.gcnArchName);
+  if (cus) *cus = p.multiProcessorCount;
+  if (platform && platform_len > 0)
+    std::snprintf(platform, platform_len, "HIP %d.%d", rv / 10000000, (rv / 100000) % 100);
+  return ATX_OK;
+}
+
+int atx_set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(ATX_E_NODEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return ATX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,679 @@
+// atx_kernel.hip -- persistent FP64 tracking kernel for gfx950 (MI355X).
+//
+// One lane tracks one particle through every element of every turn in a
+// single launch; the particle's 6-vector lives in registers from the first
+// load to the final store, and HBM is touched only at start, at refpt
+// copies, on a loss and at the end. Element parameters are compile-time
+// prepared by atx_host.cpp and read through the constant address space, so
+// the uniform per-element values land in SGPRs via scalar loads (every lane
+// of a workgroup walks the same element sequence; no divergence on the
+// element switch).
+//
+// This file is compiled twice:
+//   ATX_STRICT=1, -ffp-contract=off: the reference operation order of
+//     atintegrators/*.c, with the same divisions, so results agree bit for
+//     bit with the CPU reference except where a device transcendental (sin in
+//     RFCavityPass, tan in per-particle edge fringes) enters;
+//   ATX_STRICT=0, -ffp-contract=fast: the same maps with FMA contraction,
+//     hand-fused Horner steps and the 1/(1+delta) reciprocals hoisted where
+//     delta is constant; agreement with the reference is to rounding.
+#include <hip/hip_runtime.h>
+
+#include "atx_device.h"
+
+#ifndef ATX_STRICT
+#error "ATX_STRICT must be defined to 0 or 1"
+#endif
+
+#if ATX_STRICT
+#define ATX_NS strict_ns
+#define ATX_LAUNCH atx_launch_track_strict
+#else
+#define ATX_NS fast_ns
+#define ATX_LAUNCH atx_launch_track_fast
+#endif
+
+namespace atx {
+namespace ATX_NS {
+
+// Read-only element data: constant address space -> scalar loads.
+typedef __attribute__((address_space(4))) const double cdouble;
+typedef __attribute__((address_space(4))) const DevElem cDevElem;
+typedef __attribute__((address_space(4))) const Instance cInstance;
+typedef __attribute__((address_space(4))) const int32_t cint;
+
+struct R6 {
+  double x, px, y, py, dp, ct;
+};
+
+// Forest-Ruth 4th-order coefficients (atintegrators/atconstants.h:15-19);
+// only their products with the slice length are used, prepared on the host.
+
+__device__ __forceinline__ void add_t(R6& r, cdouble* t) {
+  r.x += t[0];
+  r.px += t[1];
+  r.y += t[2];
+  r.py += t[3];
+  r.dp += t[4];
+  r.ct += t[5];
+}
+
+// Per-element (not per-slice) maps that are large or use transcendentals are
+// kept out of line: inlined into every element variant they would multiply
+// the kernel's code size and register demand (a 6x6 matrix alone holds 72
+// SGPRs), while they run at most a few times per element.
+
+// r <- M r with M column-major (atlalib.c:17-32); the sum starts from 0.0 as
+// in the reference.
+__device__ __noinline__ R6 mul_r_v(R6 r, cdouble* m) {
+  const double v[6] = {r.x, r.px, r.y, r.py, r.dp, r.ct};
+  double o[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) s += m[i + 6 * j] * v[j];
+    o[i] = s;
+  }
+  return R6{o[0], o[1], o[2], o[3], o[4], o[5]};
+}
+
+__device__ __forceinline__ void mul_r(R6& r, cdouble* m) { r = mul_r_v(r, m); }
+
+// atlalib.c:70-75
+__device__ __noinline__ R6 change_pref_v(R6 r, double s) {
+  r.px /= s;
+  r.py /= s;
+  r.dp = (r.dp + 1.0 - s) / s;
+  return r;
+}
+
+__device__ __forceinline__ void change_pref(R6& r, double s) { r = change_pref_v(r, s); }
+
+// Apertures are decisions: evaluated without contraction in both builds so
+// that the loss element is the reference's (atlalib.c:164-190).
+__device__ __forceinline__ void rect_ap(R6& r, cdouble* lim) {
+#pragma clang fp contract(off)
+  if (r.x < lim[0] || r.x > lim[1])
+    r.ct = __builtin_inf();
+  else if (r.y < lim[2] || r.y > lim[3])
+    r.ct = __builtin_inf();
+}
+
+__device__ __forceinline__ void ellip_ap(R6& r, cdouble* ax) {
+#pragma clang fp contract(off)
+  const double u = r.x / ax[0];
+  const double v = r.y / ax[1];
+  if ((u * u + v * v) >= 1.0) r.ct = __builtin_inf();
+}
+
+__device__ __forceinline__ void apertures(R6& r, cDevElem& h, cdouble* pool) {
+  if (h.flags & F_RAP) rect_ap(r, pool + h.rap);
+  if (h.flags & F_EAP) ellip_ap(r, pool + h.eap);
+}
+
+__device__ __forceinline__ void entry_xform(R6& r, cDevElem& h, cdouble* pool) {
+  if (h.flags & F_T1) add_t(r, pool + h.t1);
+  if (h.flags & F_R1) mul_r(r, pool + h.r1);
+}
+
+__device__ __forceinline__ void exit_xform(R6& r, cDevElem& h, cdouble* pool) {
+  if (h.flags & F_R2) mul_r(r, pool + h.r2);
+  if (h.flags & F_T2) add_t(r, pool + h.t2);
+}
+
+// Full drift with its own 1/(1+delta) (atlalib.c:59-68, driftkickrad.c:36-45,
+// diff_drift.c:30-43 share this arithmetic).
+__device__ __forceinline__ void drift6(R6& r, double len) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double nl = len * pn;
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * pn * (r.px * r.px + r.py * r.py) / 2.0;
+}
+
+// In-magnet drift with nl = L/(1+delta) hoisted (driftkick.c:37-48).
+// h2 = 1/(2(1+delta)) is used by the fast build only.
+__device__ __forceinline__ void fastdrift(R6& r, double nl, double h2) {
+#if ATX_STRICT
+  (void)h2;
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * (r.px * r.px + r.py * r.py) / (2.0 * (1.0 + r.dp));
+#else
+  r.x = __builtin_fma(nl, r.px, r.x);
+  r.y = __builtin_fma(nl, r.py, r.y);
+  const double p2 = __builtin_fma(r.px, r.px, r.py * r.py);
+  r.ct = __builtin_fma(nl * p2, h2, r.ct);
+#endif
+}
+
+// Horner evaluation of sum (B_k + i A_k)(x + i y)^k (driftkick.c:92-110).
+// N >= 0: order known at compile time; N < 0: runtime order n.
+template <int N>
+__device__ __forceinline__ void field(const R6& r, cdouble* a, cdouble* b, int n, double& re,
+                                      double& im) {
+  const int order = (N >= 0) ? N : n;
+  re = b[order];
+  im = a[order];
+#pragma unroll
+  for (int k = order - 1; k >= 0; k--) {
+#if ATX_STRICT
+    const double t = re * r.x - im * r.y + b[k];
+    im = im * r.x + re * r.y + a[k];
+#else
+    const double t = __builtin_fma(re, r.x, __builtin_fma(-im, r.y, b[k]));
+    im = __builtin_fma(im, r.x, __builtin_fma(re, r.y, a[k]));
+#endif
+    re = t;
+  }
+}
+
+// driftkick.c:92-115
+template <int N>
+__device__ __forceinline__ void str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl) {
+  double re, im;
+  field<N>(r, a, b, n, re, im);
+  r.px -= kl * re;
+  r.py += kl * im;
+}
+
+// driftkick.c:51-89; kli = kl*irho prepared on the host
+template <int N>
+__device__ __forceinline__ void bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
+                                         double irho, double kli) {
+  double re, im;
+  field<N>(r, a, b, n, re, im);
+#if ATX_STRICT
+  r.px -= kl * (re - (r.dp - r.x * irho) * irho);
+  r.py += kl * im;
+  r.ct += kli * r.x;
+#else
+  const double u = __builtin_fma(-r.x, irho, r.dp);
+  const double v = __builtin_fma(-u, irho, re);
+  r.px = __builtin_fma(-kl, v, r.px);
+  r.py = __builtin_fma(kl, im, r.py);
+  r.ct = __builtin_fma(kli, r.x, r.ct);
+#endif
+}
+
+// Radiating drift (diff_drift.c:30-43). The fast build returns pn for the
+// following kick, which needs 1/(1+delta) at the same delta.
+__device__ __forceinline__ double rad_drift(R6& r, double len) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double nl = len * pn;
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * pn * (r.px * r.px + r.py * r.py) / 2.0;
+  return pn;
+}
+
+// Radiating straight kick (diff_str_kick.c:19-83, bdiff == NULL).
+template <int N>
+__device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
+                                             double radc, double pn_in) {
+  double re, im;
+  field<N>(r, a, b, n, re, im);
+#if ATX_STRICT
+  (void)pn_in;
+  double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double vn2 = 1.0 / (1.0 + xpr * xpr + ypr * ypr);
+  const double cr = im * ypr - re * xpr;
+  const double b2p = (re * re + im * im + cr * cr) * vn2;
+  const double factor = (1.0 + (xpr * xpr + ypr * ypr) / 2.0) / (pn * pn) * kl;
+  r.dp -= radc * b2p * factor;
+  pn = 1.0 / (1.0 + r.dp);
+  r.px = xpr / pn;
+  r.py = ypr / pn;
+  r.px -= kl * re;
+  r.py += kl * im;
+#else
+  const double opd = 1.0 + r.dp;
+  const double xpr = r.px * pn_in;
+  const double ypr = r.py * pn_in;
+  const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
+  const double vn2 = 1.0 / (1.0 + s2);
+  const double cr = __builtin_fma(im, ypr, -re * xpr);
+  const double b2p = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im)) * vn2;
+  const double factor = __builtin_fma(0.5, s2, 1.0) * (opd * opd) * kl;
+  r.dp = __builtin_fma(-radc * b2p, factor, r.dp);
+  const double opd2 = 1.0 + r.dp;
+  r.px = __builtin_fma(xpr, opd2, -kl * re);
+  r.py = __builtin_fma(ypr, opd2, kl * im);
+#endif
+}
+
+// Radiating curved-frame kick (diff_bnd_kick.c:85-174, bdiff == NULL).
+template <int N>
+__device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
+                                             double irho, double kli, double radc,
+                                             double pn_in) {
+  double re, im;
+  field<N>(r, a, b, n, re, im);
+#if ATX_STRICT
+  (void)pn_in;
+  double pn = 1.0 / (1.0 + r.dp);
+  const double x = r.x;
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double by = re + irho;
+  const double bx = im;
+  const double hx = 1.0 + x * irho;
+  const double vn2 = 1.0 / (hx * hx + xpr * xpr + ypr * ypr);
+  const double byh = by * hx;
+  const double bxh = bx * hx;
+  const double cr = bx * ypr - by * xpr;
+  const double b2p = (byh * byh + bxh * bxh + cr * cr) * vn2;
+  const double factor = (1.0 + x * irho + (xpr * xpr + ypr * ypr) / 2.0) / (pn * pn) * kl;
+  const double d0 = r.dp;
+  r.dp -= radc * b2p * factor;
+  pn = 1.0 / (1.0 + r.dp);
+  r.px = xpr / pn;
+  r.py = ypr / pn;
+  r.px -= kl * (re - (d0 - r.x * irho) * irho);
+  r.py += kl * im;
+  r.ct += kli * r.x;
+#else
+  const double x = r.x;
+  const double opd = 1.0 + r.dp;
+  const double xpr = r.px * pn_in;
+  const double ypr = r.py * pn_in;
+  const double by = re + irho;
+  const double hx = __builtin_fma(x, irho, 1.0);
+  const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
+  const double vn2 = 1.0 / __builtin_fma(hx, hx, s2);
+  const double byh = by * hx;
+  const double bxh = im * hx;
+  const double cr = __builtin_fma(im, ypr, -by * xpr);
+  const double b2p = __builtin_fma(cr, cr, __builtin_fma(byh, byh, bxh * bxh)) * vn2;
+  const double factor = __builtin_fma(0.5, s2, hx) * (opd * opd) * kl;
+  const double d0 = r.dp;
+  r.dp = __builtin_fma(-radc * b2p, factor, r.dp);
+  const double opd2 = 1.0 + r.dp;
+  const double u = __builtin_fma(-x, irho, d0);
+  const double v = __builtin_fma(-u, irho, re);
+  r.px = __builtin_fma(xpr, opd2, -kl * v);
+  r.py = __builtin_fma(ypr, opd2, kl * im);
+  r.ct = __builtin_fma(kli, x, r.ct);
+#endif
+}
+
+// Edge focusing (atphyslib.c:15-79, diff_bend_fringe.c:2-53). sgn = +1 at
+// the entrance, -1 at the exit; fx, c0 = irho*tan(edge), edge and corr/psi
+// come from the scalar block.
+__device__ __noinline__ double edge_fy_tan(int mode, double dp, double px, double irho,
+                                           double edge_angle, double corr, double sgn) {
+  switch (mode) {
+    case E_TAN1:
+      return irho * tan(edge_angle - corr / (1.0 + dp));
+    case E_TAN2:
+      return irho * tan(edge_angle - corr / (1.0 + dp)) / (1.0 + dp);
+    case E_TAN3:
+      return (sgn > 0.0) ? irho * tan(edge_angle - corr + px / (1.0 + dp))
+                         : irho * tan(edge_angle - corr - px / (1.0 + dp));
+    case E_RTAN1: {
+      const double pn = 1.0 / (1.0 + dp);
+      return irho * tan(edge_angle - corr * pn);
+    }
+    case E_RTAN2: {
+      const double pn = 1.0 / (1.0 + dp);
+      return irho * tan(edge_angle - corr * pn) * pn;
+    }
+    default: {  // E_RTAN3
+      const double pn = 1.0 / (1.0 + dp);
+      return irho * tan(edge_angle - corr + sgn * px * pn);
+    }
+  }
+}
+
+__device__ __forceinline__ void edge(R6& r, int mode, double irho, double fx, double c0,
+                                     double edge_angle, double corr, double sgn) {
+  double fy;
+  if (mode == E_CONST)
+    fy = c0;
+  else if (mode == E_DIVP)
+    fy = c0 / (1.0 + r.dp);
+  else if (mode == E_MULPN)
+    fy = c0 * (1.0 / (1.0 + r.dp));
+  else
+    fy = edge_fy_tan(mode, r.dp, r.px, irho, edge_angle, corr, sgn);
+  r.px += r.x * fx;
+  r.py -= r.y * fy;
+}
+
+// Lee-Whiting hard-edge quadrupole fringe (quadfringe.c:3-51).
+__device__ __forceinline__ void quad_fringe_inl(R6& r, double b2, bool entrance) {
+  const double u = b2 / (12.0 * (1.0 + r.dp));
+  const double x2 = r.x * r.x;
+  const double z2 = r.y * r.y;
+  const double xz = r.x * r.y;
+  const double gx = u * (x2 + 3.0 * z2) * r.x;
+  const double gz = u * (z2 + 3.0 * x2) * r.y;
+  const double dpx = 3.0 * u * (2.0 * xz * r.py - (x2 + z2) * r.px);
+  const double dpy = 3.0 * u * (2.0 * xz * r.px - (x2 + z2) * r.py);
+  if (entrance) {
+    r.x += gx;
+    r.y -= gz;
+    r.ct -= (gz * r.py - gx * r.px) / (1.0 + r.dp);
+    r.px += dpx;
+    r.py -= dpy;
+  } else {
+    r.x -= gx;
+    r.y += gz;
+    r.ct += (gz * r.py - gx * r.px) / (1.0 + r.dp);
+    r.px -= dpx;
+    r.py += dpy;
+  }
+}
+
+// 2x2 linear blocks of the elegant fringe (quadfringe.c:93-163), matrices
+// prepared on the host: m = {Rxx, Rxp, Rpx, Rpp, Ryy, Ryq, Rqy, Rqq}.
+__device__ __forceinline__ void apply_2x2(R6& r, cdouble* m) {
+  const double x = r.x, px = r.px, y = r.y, py = r.py;
+  r.x = m[0] * x + m[1] * px;
+  r.px = m[2] * x + m[3] * px;
+  r.y = m[4] * y + m[5] * py;
+  r.py = m[6] * y + m[7] * py;
+}
+
+// mode: Q_LEE or Q_ELEGANT; m: the two 2x2 blocks around the Lee-Whiting map
+__device__ __noinline__ R6 quad_fringe_v(R6 r, int mode, cdouble* m, double b2, bool entrance) {
+  if (mode == Q_ELEGANT) apply_2x2(r, m);
+  quad_fringe_inl(r, b2, entrance);
+  if (mode == Q_ELEGANT) apply_2x2(r, m + 8);
+  return r;
+}
+
+__device__ __forceinline__ void quad_fringe_in(R6& r, cDevElem& h, cdouble* pool, double b2) {
+  r = quad_fringe_v(r, h.fringe & 0xff, pool + h.fq, b2, true);
+}
+
+__device__ __forceinline__ void quad_fringe_out(R6& r, cDevElem& h, cdouble* pool, double b2) {
+  r = quad_fringe_v(r, (h.fringe >> 8) & 0xff, pool + h.fq + 16, b2, false);
+}
+
+// StrMPoleSymplectic4Pass.c:57-104 and BndMPoleSymplectic4Pass.c:75-122.
+template <bool BEND, int N>
+__device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
+  cdouble* s = pool + h.sc;
+  cdouble* a = pool + h.poly;
+  cdouble* b = a + h.order + 1;
+  const int n = h.order;
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double nl1 = s[S_L1] * pn;
+  const double nl2 = s[S_L2] * pn;
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  const double irho = s[S_IRHO];
+  if (BEND) edge(r, h.edge_in, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
+  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
+  const double k1 = s[S_K1], k2 = s[S_K2];
+  const double ki1 = k1 * irho, ki2 = k2 * irho;
+#if ATX_STRICT
+  const double h2 = 0.0;
+#else
+  const double h2 = 0.5 / (1.0 + r.dp);  // delta is constant through the slices
+#endif
+  for (int m = 0; m < h.nslice; m++) {
+    fastdrift(r, nl1, h2);
+    if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+    fastdrift(r, nl2, h2);
+    if (BEND) bnd_kick<N>(r, a, b, n, k2, irho, ki2); else str_kick<N>(r, a, b, n, k2);
+    fastdrift(r, nl2, h2);
+    if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+    fastdrift(r, nl1, h2);
+  }
+  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
+  if (BEND) edge(r, h.edge_out, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
+  apertures(r, h, pool);
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
+// StrMPoleSymplectic4RadPass.c:61-110 and BndMPoleSymplectic4RadPass.c:76-123.
+template <bool BEND, int N>
+__device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
+  cdouble* s = pool + h.sc;
+  cdouble* a = pool + h.poly;
+  cdouble* b = a + h.order + 1;
+  const int n = h.order;
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  const double irho = s[S_IRHO];
+  if (BEND) edge(r, h.edge_in, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
+  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
+  const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
+  const double ki1 = k1 * irho, ki2 = k2 * irho;
+  const double radc = s[S_RADC];
+  for (int m = 0; m < h.nslice; m++) {
+    double pn = rad_drift(r, l1);
+    if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+    else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+    pn = rad_drift(r, l2);
+    if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
+    else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
+    pn = rad_drift(r, l2);
+    if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+    else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+    rad_drift(r, l1);
+  }
+  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
+  if (BEND) edge(r, h.edge_out, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
+  apertures(r, h, pool);
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
+template <bool BEND, bool RAD>
+__device__ __forceinline__ void mpole_dispatch(R6& r, cDevElem& h, cdouble* pool) {
+  switch (h.order) {
+    case 0: RAD ? mpole_rad<BEND, 0>(r, h, pool) : mpole<BEND, 0>(r, h, pool); break;
+    case 1: RAD ? mpole_rad<BEND, 1>(r, h, pool) : mpole<BEND, 1>(r, h, pool); break;
+    case 2: RAD ? mpole_rad<BEND, 2>(r, h, pool) : mpole<BEND, 2>(r, h, pool); break;
+    case 3: RAD ? mpole_rad<BEND, 3>(r, h, pool) : mpole<BEND, 3>(r, h, pool); break;
+    default: RAD ? mpole_rad<BEND, -1>(r, h, pool) : mpole<BEND, -1>(r, h, pool); break;
+  }
+}
+
+// ThinMPolePass.c:42-63
+__device__ __forceinline__ void thin(R6& r, cDevElem& h, cdouble* pool) {
+  cdouble* s = pool + h.sc;
+  cdouble* a = pool + h.poly;
+  cdouble* b = a + h.order + 1;
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  switch (h.order) {
+    case 0: str_kick<0>(r, a, b, 0, 1.0); break;
+    case 1: str_kick<1>(r, a, b, 1, 1.0); break;
+    case 2: str_kick<2>(r, a, b, 2, 1.0); break;
+    case 3: str_kick<3>(r, a, b, 3, 1.0); break;
+    default: str_kick<-1>(r, a, b, h.order, 1.0); break;
+  }
+  const double bax = s[S_BAX], bay = s[S_BAY];
+  r.px += bax * r.dp;
+  r.py -= bay * r.dp;
+  r.ct -= bax * r.x - bay * r.y;
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
+// attrackfunc.c:5-42: hft = h/f - T0 prepared on the host.
+__device__ __noinline__ double rf_kick(double ct, double nv, double twopif, double lag,
+                                       double hft0, double philag, double nt) {
+  return -nv * sin(twopif * ((ct - lag) / 2.99792458e8 - hft0 * nt) - philag);
+}
+
+__device__ __forceinline__ void rf_cavity(R6& r, cDevElem& h, cdouble* pool, int nturn) {
+  cdouble* s = pool + h.sc;
+  const double nt = (double)nturn;
+  if (!(h.flags & F_THICK)) {
+    if (h.flags & F_RFKICK)
+      r.dp += rf_kick(r.ct, s[S_RF_NV], s[S_RF_TWOPIF], s[S_RF_LAG], s[S_RF_HFT0], s[S_RF_PHILAG], nt);
+  } else {
+    drift6(r, s[S_RF_HALF]);
+    if (h.flags & F_RFKICK)
+      r.dp += rf_kick(r.ct, s[S_RF_NV], s[S_RF_TWOPIF], s[S_RF_LAG], s[S_RF_HFT0], s[S_RF_PHILAG], nt);
+    drift6(r, s[S_RF_HALF]);
+  }
+}
+
+// CorrectorPass.c:25-90
+__device__ __forceinline__ void corrector(R6& r, cDevElem& h, cdouble* pool) {
+  cdouble* s = pool + h.sc;
+  const double xk = s[S_C_XK], yk = s[S_C_YK];
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  if (!(h.flags & F_THICK)) {
+    entry_xform(r, h, pool);
+    apertures(r, h, pool);
+    r.px += xk;
+    r.py += yk;
+  } else {
+    const double pn = 1.0 / (1.0 + r.dp);
+    const double nl = s[S_C_LEN] * pn;
+    entry_xform(r, h, pool);
+    apertures(r, h, pool);
+    r.ct += nl * pn * (xk * xk / 3.0 + yk * yk / 3.0 + r.px * r.px + r.py * r.py + r.px * xk + r.py * yk) / 2.0;
+    r.x += nl * (r.px + xk / 2.0);
+    r.px += xk;
+    r.y += nl * (r.py + yk / 2.0);
+    r.py += yk;
+    apertures(r, h, pool);
+  }
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
+__device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int nturn) {
+  switch (h.kind) {
+    case K_IDENTITY:  // IdentityPass.c:20-43
+      entry_xform(r, h, pool);
+      apertures(r, h, pool);
+      exit_xform(r, h, pool);
+      break;
+    case K_DRIFT:  // DriftPass.c:16-48
+      entry_xform(r, h, pool);
+      apertures(r, h, pool);
+      drift6(r, pool[h.sc + S_D_LEN]);
+      apertures(r, h, pool);
+      exit_xform(r, h, pool);
+      break;
+    case K_STR: mpole_dispatch<false, false>(r, h, pool); break;
+    case K_BND: mpole_dispatch<true, false>(r, h, pool); break;
+    case K_STRRAD: mpole_dispatch<false, true>(r, h, pool); break;
+    case K_BNDRAD: mpole_dispatch<true, true>(r, h, pool); break;
+    case K_THIN: thin(r, h, pool); break;
+    case K_RF: rf_cavity(r, h, pool, nturn); break;
+    case K_CORR: corrector(r, h, pool); break;
+    case K_APERTURE: rect_ap(r, pool + h.rap); break;  // AperturePass.c:9-26
+    default: break;
+  }
+}
+
+// at.c:115-139 predicate: any coordinate not finite, or |r_n| > 1 for n < 5.
+__device__ __forceinline__ bool is_lost(const R6& r) {
+  return !(__builtin_fabs(r.x) <= 1.0) || !(__builtin_fabs(r.px) <= 1.0) ||
+         !(__builtin_fabs(r.y) <= 1.0) || !(__builtin_fabs(r.py) <= 1.0) ||
+         !(__builtin_fabs(r.dp) <= 1.0) || !(__builtin_fabs(r.ct) < __builtin_inf());
+}
+
+__device__ __forceinline__ void store6(double* base, int64_t cs, const R6& r) {
+  base[0] = r.x;
+  base[cs] = r.px;
+  base[2 * cs] = r.y;
+  base[3 * cs] = r.py;
+  base[4 * cs] = r.dp;
+  base[5 * cs] = r.ct;
+}
+
+__global__ void __launch_bounds__(256) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
+  cInstance* insts = (cInstance*)insts_g;
+  // Instance of this workgroup: the last whose first block is <= blockIdx.x.
+  int lo = 0, hi = ninst - 1;
+  const int bid = (int)blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (insts[mid].block_begin <= bid) lo = mid; else hi = mid - 1;
+  }
+  cInstance& I = insts[lo];
+  const int p = (bid - I.block_begin) * (int)blockDim.x + (int)threadIdx.x;
+  const int npart = I.npart;
+  if (p >= npart) return;
+
+  cDevElem* elems = (cDevElem*)I.elems;
+  cdouble* pool = (cdouble*)I.pool;
+  cint* slots = (cint*)I.slots;
+  double* rp = I.r + (int64_t)p * I.pstride;
+  const int64_t cs = I.cstride;
+  R6 r;
+  r.x = rp[0];
+  r.px = rp[cs];
+  r.y = rp[2 * cs];
+  r.py = rp[3 * cs];
+  r.dp = rp[4 * cs];
+  r.ct = rp[5 * cs];
+
+  const int nelem = I.nelem;
+  const int losses = I.losses;
+  const bool check = !I.elempass;
+  double* rout = I.rout;
+  const int64_t np6 = 6 * (int64_t)npart;
+  if (losses) {
+    I.lost_flag[p] = 0;
+    I.lost_turn[p] = 0;
+    I.lost_elem[p] = 0;
+    R6 z = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    store6(I.lost_coord + 6 * (int64_t)p, 1, z);
+  }
+  bool lost = false;
+  const int nturns = I.nturns;
+  for (int t = 0; t < nturns; t++) {
+    const int nturn = I.turn0 + t;
+    const int e0 = (t == 0) ? I.start_elem : 0;
+    cint* slot = slots + ((t == 0) ? 0 : nelem + 1);
+    const int64_t base = (t == 0) ? 0 : (int64_t)I.n_first + (int64_t)(t - 1) * I.n_std;
+    const int e1 = I.end_elem;
+    for (int e = e0; e < e1; e++) {
+      const int sl = slot[e];
+      if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+      cDevElem& h = elems[e];
+      if (!__builtin_isnan(r.x)) element(r, h, pool, nturn);
+      if (check) {
+        if (losses) {
+          if (!lost && is_lost(r)) {
+            lost = true;
+            I.lost_flag[p] = 1;
+            I.lost_turn[p] = (uint32_t)nturn;
+            I.lost_elem[p] = (uint32_t)e;
+            store6(I.lost_coord + 6 * (int64_t)p, 1, r);
+            r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
+          }
+        } else if (__builtin_isfinite(r.x) && is_lost(r)) {
+          r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};  // at.c:142-161
+        }
+      }
+    }
+    const int sl = slot[nelem];
+    if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+  }
+  rp[0] = r.x;
+  rp[cs] = r.px;
+  rp[2 * cs] = r.y;
+  rp[3 * cs] = r.py;
+  rp[4 * cs] = r.dp;
+  rp[5 * cs] = r.ct;
+}
+
+}  // namespace ATX_NS
+}  // namespace atx
+
+extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
+                                 hipStream_t stream) {
+  hipLaunchKernelGGL(atx::ATX_NS::track_kernel, dim3(nblocks), dim3(block), 0, stream, d_insts,
+                     ninst);
+  return hipGetLastError();
+}

@@ -1,0 +1,256 @@
+"""ctypes binding of libatx.so (C ABI in include/atx.h).
+
+``DeviceLattice`` turns element objects into an engine lattice handle by
+setting every numeric attribute by name, exactly as atgpu's PyInterface
+feeds PassMethodFactory (atgpu/PyInterface.cpp:404-480). The library is the
+only compute path: importing this module raises when it is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from .lattice import AtError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("ATX_LIBRARY", os.path.join(_HERE, "libatx.so"))
+
+ATX_OK = 0
+ATX_E_VALUE = 1
+ATX_E_ATTRIBUTE = 2
+ATX_E_RUNTIME = 3
+ATX_E_NODEVICE = 4
+
+ATX_DEVICE_POINTERS = 0x1
+ATX_STRICT = 0x2
+ATX_KEEP_COUNTER = 0x4
+ATX_SOA = 0x8
+
+# Exported symbols, as declared in include/atx.h.
+EXPORTS = (
+    "atx_last_error", "atx_lattice_create", "atx_lattice_destroy", "atx_lattice_add_element",
+    "atx_element_set_double", "atx_element_set_int", "atx_element_set_array",
+    "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
+    "atx_lattice_turn_counter", "atx_device_count", "atx_device_info", "atx_set_device",
+)
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: str | None = None) -> C.CDLL:
+    """Load libatx.so and declare its signatures. Raises when it is absent:
+    there is no fallback compute path."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise ImportError(f"at_amd engine library not built: {p} (run python -m at_amd.build)")
+        lib = C.CDLL(p)
+        lib.atx_last_error.restype = C.c_char_p
+        lib.atx_lattice_create.argtypes = [C.POINTER(C.c_void_p)]
+        lib.atx_lattice_destroy.argtypes = [C.c_void_p]
+        lib.atx_lattice_add_element.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
+        lib.atx_element_set_double.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        lib.atx_element_set_int.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
+        lib.atx_element_set_array.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int64]
+        lib.atx_lattice_num_elements.argtypes = [C.c_void_p]
+        lib.atx_lattice_check.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        lib.atx_track.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, _u32p, C.c_uint32,
+                                  C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                  C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+        lib.atx_elempass.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_double,
+                                     C.c_double, C.c_void_p, C.c_uint32]
+        lib.atx_lattice_turn_counter.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.atx_device_count.argtypes = [C.POINTER(C.c_int)]
+        lib.atx_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
+                                        C.POINTER(C.c_int), C.c_char_p, C.c_int]
+        lib.atx_set_device.argtypes = [C.c_int]
+        for name in EXPORTS:
+            getattr(lib, name).restype = C.c_char_p if name == "atx_last_error" else C.c_int
+        _lib = lib
+        return lib
+
+
+def _raise(rc: int, lib) -> None:
+    msg = lib.atx_last_error().decode(errors="replace")
+    if rc == ATX_E_VALUE:
+        raise ValueError(msg)
+    if rc == ATX_E_ATTRIBUTE:
+        raise AttributeError(msg)
+    raise RuntimeError(msg)
+
+
+def _check(rc: int, lib) -> None:
+    if rc != ATX_OK:
+        _raise(rc, lib)
+
+
+# Attributes that are never sent to the engine (names and bookkeeping).
+_SKIP = {"FamName", "PassMethod", "Class"}
+
+
+def _send_attribute(lib, h, name: str, value) -> None:
+    if name in _SKIP or name.startswith("_"):
+        return
+    bname = name.encode()
+    if isinstance(value, (bool, np.bool_)):
+        _check(lib.atx_element_set_int(h, bname, int(value)), lib)
+    elif isinstance(value, (int, np.integer)):
+        _check(lib.atx_element_set_int(h, bname, int(value)), lib)
+    elif isinstance(value, (float, np.floating)):
+        _check(lib.atx_element_set_double(h, bname, float(value)), lib)
+    elif isinstance(value, np.ndarray) and value.dtype.kind in "fiub":
+        a = np.asarray(value, dtype=np.float64)
+        a = np.ascontiguousarray(a.ravel(order="F"))
+        _check(lib.atx_element_set_array(h, bname, a.ctypes.data_as(_dp), a.size), lib)
+    elif isinstance(value, (list, tuple)) and all(isinstance(v, (int, float)) for v in value):
+        a = np.ascontiguousarray(np.asarray(value, dtype=np.float64))
+        _check(lib.atx_element_set_array(h, bname, a.ctypes.data_as(_dp), a.size), lib)
+    # other attribute types (strings, objects) carry no pass-method parameter
+
+
+class DeviceLattice:
+    """Engine lattice handle built from a sequence of elements."""
+
+    def __init__(self, elements):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.atx_lattice_create(C.byref(h)), self.lib)
+        self.h = h
+        self.nelem = 0
+        try:
+            for i, e in enumerate(elements):
+                pm = getattr(e, "PassMethod", None)
+                if pm is None:
+                    raise AttributeError(f"Element #{i} has no PassMethod")
+                fam = str(getattr(e, "FamName", ""))
+                rc = self.lib.atx_lattice_add_element(h, pm.encode(), fam.encode())
+                if rc != ATX_OK:
+                    _raise(rc, self.lib)
+                for name, value in vars(e).items():
+                    _send_attribute(self.lib, h, name, value)
+                self.nelem += 1
+        except Exception:
+            self.close()
+            raise
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.atx_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, energy: float, rest_energy: float) -> None:
+        _check(self.lib.atx_lattice_check(self.h, float(energy), float(rest_energy)), self.lib)
+
+    @property
+    def turn_counter(self) -> int:
+        t = C.c_int64()
+        _check(self.lib.atx_lattice_turn_counter(self.h, C.byref(t)), self.lib)
+        return int(t.value)
+
+    def track(self, r_in: np.ndarray, nturns: int, refpts=None, turn: int = 0,
+              energy: float = 0.0, rest_energy: float = 0.0, charge: float = -1.0,
+              losses: bool = False, start_elem: int = 0, end_elem: int = 0,
+              keep_counter: bool = False, strict: bool = False):
+        """Host-array tracking with the at.c contract; returns rout or
+        (rout, loss dict)."""
+        if r_in.ndim not in (1, 2) or r_in.shape[0] != 6:
+            raise ValueError("rin is not 6D")
+        if r_in.dtype != np.float64:
+            raise ValueError("rin is not a double array")
+        if not r_in.flags.f_contiguous:
+            raise ValueError("rin is not Fortran-aligned")
+        npart = r_in.size // 6
+        refs = np.zeros(0, np.uint32) if refpts is None else np.ascontiguousarray(refpts, np.uint32)
+        rout = np.full((6, npart, refs.size, max(nturns, 0)), np.nan, order="F")
+        flags = (ATX_STRICT if strict else 0) | (ATX_KEEP_COUNTER if keep_counter else 0)
+        lf = lt = le = lc = None
+        if losses:
+            lf = np.zeros(npart, np.uint8)
+            lt = np.zeros(npart, np.uint32)
+            le = np.zeros(npart, np.uint32)
+            lc = np.zeros((6, npart), order="F")
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        rc = self.lib.atx_track(self.h, ptr(r_in), npart, int(nturns),
+                                refs.ctypes.data_as(_u32p), refs.size, int(turn), float(energy),
+                                float(rest_energy), float(charge), int(bool(losses)),
+                                int(start_elem), int(end_elem), ptr(rout) if rout.size else None,
+                                ptr(lf), ptr(lt), ptr(le), ptr(lc), None, flags)
+        _check(rc, self.lib)
+        if losses:
+            return rout, {"islost": lf.astype(bool), "turn": lt, "elem": le, "coord": lc}
+        return rout
+
+    def track_device(self, r_ptr: int, npart: int, nturns: int, *, refpts=None,
+                     rout_ptr: int | None = None, turn: int = 0, energy: float = 0.0,
+                     rest_energy: float = 0.0, losses: bool = False, loss_ptrs=None,
+                     stream: int | None = None, strict: bool = False, soa: bool = False,
+                     start_elem: int = 0, end_elem: int = 0) -> None:
+        """Asynchronous tracking of device-resident particles (raw device
+        pointers, e.g. torch ``tensor.data_ptr()``) on a HIP stream."""
+        refs = np.zeros(0, np.uint32) if refpts is None else np.ascontiguousarray(refpts, np.uint32)
+        flags = ATX_DEVICE_POINTERS | (ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0)
+        lf, lt, le, lc = loss_ptrs if loss_ptrs is not None else (None, None, None, None)
+        rc = self.lib.atx_track(self.h, r_ptr, int(npart), int(nturns),
+                                refs.ctypes.data_as(_u32p), refs.size, int(turn), float(energy),
+                                float(rest_energy), -1.0, int(bool(losses)), int(start_elem),
+                                int(end_elem), rout_ptr, lf, lt, le, lc, stream, flags)
+        _check(rc, self.lib)
+
+    def elempass(self, index: int, r_in: np.ndarray, energy: float = 0.0,
+                 rest_energy: float = 0.0, strict: bool = False) -> np.ndarray:
+        if r_in.shape[0] != 6 or r_in.dtype != np.float64 or not r_in.flags.f_contiguous:
+            raise ValueError("rin must be a 6xN float64 Fortran-ordered array")
+        rc = self.lib.atx_elempass(self.h, int(index), r_in.ctypes.data, r_in.size // 6,
+                                   float(energy), float(rest_energy), None,
+                                   ATX_STRICT if strict else 0)
+        _check(rc, self.lib)
+        return r_in
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int(0)
+    rc = lib.atx_device_count(C.byref(n))
+    return int(n.value) if rc == ATX_OK else 0
+
+
+def device_info(dev: int) -> list:
+    lib = load_library()
+    name = C.create_string_buffer(256)
+    arch = C.create_string_buffer(64)
+    plat = C.create_string_buffer(64)
+    cus = C.c_int(0)
+    _check(lib.atx_device_info(dev, name, 256, arch, 64, C.byref(cus), plat, 64), lib)
+    return [name.value.decode(), arch.value.decode(), int(cus.value), plat.value.decode()]
+
+
+def set_device(dev: int) -> None:
+    lib = load_library()
+    _check(lib.atx_set_device(int(dev)), lib)
+
+
+def require_library() -> None:
+    """Raise AtError if the engine is unusable."""
+    try:
+        load_library()
+    except ImportError as exc:
+        raise AtError(str(exc)) from exc

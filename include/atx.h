@@ -1,0 +1,115 @@
+/*
+ * atx.h -- C ABI of the MI355X tracking engine (libatx.so).
+ *
+ * The engine replaces the GPU tracking backend of AT: the Python extension
+ * at.tracking.gpu (atgpu/PyInterface.cpp:110-402, stub
+ * pyat/at/tracking/gpu.pyi:9-28) that pyat's lattice_track/internal_lpass
+ * call when use_mp=MPMode.GPU (pyat/at/tracking/track.py:115-127). It keeps
+ * the reference plugin surface: pass methods are named as in
+ * PassMethodFactory (atgpu/PassMethodFactory.cpp:33-46) and their
+ * parameters are given by attribute name, with the names, required/optional
+ * status and defaults of the CPU integrators (atintegrators/<Pass>.c
+ * trackFunction getters).
+ *
+ * All entry points return ATX_OK (0) or an error code; atx_last_error()
+ * returns the message of the last failing call on the calling thread. The
+ * codes map onto the Python exceptions pyat raises for the same conditions
+ * (pyat/at.c:409-416,491-494,514-536; atgpu/PyInterface.cpp:292-297).
+ *
+ * Arrays use the reference layouts: particle coordinates are (6, N)
+ * Fortran-ordered doubles (coordinate index fastest, pyat/at.c:409-418), refpt
+ * output is (6, N, R, T) Fortran-ordered (at.c:567-572), matrices R1/R2 are
+ * 6x6 column-major (atlalib.c:17-32).
+ */
+#ifndef ATX_H
+#define ATX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATX_OK 0
+#define ATX_E_VALUE 1     /* ValueError: shape, range, partial-turn or energy errors */
+#define ATX_E_ATTRIBUTE 2 /* AttributeError: a required element attribute is missing */
+#define ATX_E_RUNTIME 3   /* RuntimeError: unknown PassMethod, device or launch failure */
+#define ATX_E_NODEVICE 4  /* no HIP device available */
+
+/* atx_track flags */
+#define ATX_DEVICE_POINTERS 0x1u /* all arrays are device pointers; run asynchronously on stream */
+#define ATX_STRICT 0x2u          /* reference operation order, no FMA contraction */
+#define ATX_KEEP_COUNTER 0x4u    /* continue from the turn counter saved by the previous call */
+#define ATX_SOA 0x8u             /* with ATX_DEVICE_POINTERS: r_in is SoA (x[N], px[N], ...) */
+
+typedef struct atx_lattice atx_lattice;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *atx_last_error(void);
+
+/* --- Lattice construction ------------------------------------------------
+ * Replaces Lattice::addElement -> PassMethodFactory::createElement ->
+ * <Pass>::getParameters(AbstractInterface*) (atgpu/Lattice.cpp:37-56,
+ * atgpu/PassMethodFactory.cpp:73-94, atgpu/AbstractInterface.h:9-43).
+ * Attributes are set on the element added last. An unknown pass method is
+ * rejected when added ("Not implemented PassMethod: X", Lattice.cpp:45-54);
+ * missing required attributes are reported by atx_track/atx_lattice_check.
+ */
+int atx_lattice_create(atx_lattice **out);
+int atx_lattice_destroy(atx_lattice *lat);
+int atx_lattice_add_element(atx_lattice *lat, const char *pass_method, const char *fam_name);
+int atx_element_set_double(atx_lattice *lat, const char *name, double value);
+int atx_element_set_int(atx_lattice *lat, const char *name, int64_t value);
+int atx_element_set_array(atx_lattice *lat, const char *name, const double *data, int64_t n);
+int atx_lattice_num_elements(const atx_lattice *lat);
+/* Validate every element against its pass method (required attributes,
+ * array sizes, energy where radiation or RF needs it) without tracking. */
+int atx_lattice_check(atx_lattice *lat, double energy, double rest_energy);
+
+/* --- Tracking ------------------------------------------------------------
+ * Replaces at_gpupass (atgpu/PyInterface.cpp:132-402) with the loss and turn
+ * semantics of the CPU driver at_atpass (pyat/at.c:340-681):
+ *   r_in       (6,N) F-order, updated in place with the final coordinates;
+ *   refpts     sorted uint32 element indices (0 = entrance of the first
+ *              element, nelem = exit of the last), copies written to r_out
+ *              (6,N,R,T) in the order at.c writes them; r_out may be NULL
+ *              when num_refpts == 0;
+ *   turn       starting value of the turn counter (ignored with
+ *              ATX_KEEP_COUNTER); the counter enters RFCavityPass and the
+ *              loss turn;
+ *   energy     ring energy [eV]; rest_energy [eV] (0: relativistic);
+ *   losses     non-zero: fill lost_flag/lost_turn/lost_elem/lost_coord
+ *              (N, N, N, (6,N)); survivors keep 0 (at.c:574-594);
+ *   start_elem/end_elem  partial-turn tracking (end_elem 0 = all);
+ *   stream     hipStream_t (NULL = default stream);
+ *   flags      ATX_* flags above.
+ * Without ATX_DEVICE_POINTERS the arrays are host memory and the call
+ * returns when the results are back on the host. charge is accepted for
+ * interface parity and unused by the in-scope pass methods. */
+int atx_track(atx_lattice *lat, double *r_in, uint32_t num_particles, int32_t nturns,
+              const uint32_t *refpts, uint32_t num_refpts, int32_t turn, double energy,
+              double rest_energy, double charge, int32_t losses, int32_t start_elem,
+              int32_t end_elem, double *r_out, uint8_t *lost_flag, uint32_t *lost_turn,
+              uint32_t *lost_elem, double *lost_coord, void *stream, uint32_t flags);
+
+/* Single-element pass of element `index` with the semantics of at_elempass
+ * (pyat/at.c:683-753): turn 0, ring length 0, no loss check. */
+int atx_elempass(atx_lattice *lat, int32_t index, double *r_in, uint32_t num_particles,
+                 double energy, double rest_energy, void *stream, uint32_t flags);
+
+/* Turn counter after the last call (at.c:656 last_turn). */
+int atx_lattice_turn_counter(const atx_lattice *lat, int64_t *turn);
+
+/* --- Devices -------------------------------------------------------------
+ * Replaces at_gpuinfo (atgpu/PyInterface.cpp:110-130): one record per
+ * device: name, architecture (gfx name), compute-unit count, platform. */
+int atx_device_count(int *count);
+int atx_device_info(int device, char *name, int name_len, char *arch, int arch_len,
+                    int *compute_units, char *platform, int platform_len);
+int atx_set_device(int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATX_H */
