@@ -435,6 +435,11 @@ def load_mat(filename: str, key: str | None = None, **kwargs) -> Lattice:
             attrs[name] = v
         cls_name = attrs.pop("Class", None)
         fam = attrs.pop("FamName", "")
+        if cls_name == "RingParam":
+            # ring parameters, not an element (pyat/at/load/matfile.py:163-220)
+            energy = float(attrs.get("Energy", energy))
+            periodicity = int(attrs.get("Periodicity", periodicity))
+            continue
         # K and H are views of PolynomB[1]/[2] in pyat, not separate data
         attrs.pop("K", None)
         attrs.pop("H", None)

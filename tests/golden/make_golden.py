@@ -1,0 +1,112 @@
+"""Write the golden vectors of the tracking path from the reference.
+
+    python tests/golden/make_golden.py
+
+Runs the reference integrator cores compiled from /root/reference
+(oracle/_ref, built by `make -C oracle ref`) through the at.c-equivalent
+driver (oracle/ref_driver.c) and stores inputs and outputs as small .npz
+fixtures next to this script:
+
+  zoo.npz        every case of tests/zoo.py: single-element pass (elempass
+                 semantics) and a 2-element, 3-turn lattice with refpts
+                 [0, 1, 2], losses and a starting turn of 3;
+  fodo.npz       the 32-cell FODO ring, 64 particles, 10 turns, refpts;
+  large_ring.npz the large ring (radiation, RF, apertures), 48 particles
+                 from the sigma x 10 beam, 3 turns, losses;
+  hmba.npz       the reference fixture machine_data/hmba.mat: partial-turn
+                 tracking (start_elem=100; start_elem=100, end_elem=110)
+                 with the inputs of pyat/test/test_atpass.py:157-171, and
+                 8 particles x 5 turns with refpts.
+
+Only this script runs reference code; the fixtures are data.
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import pyoracle as po  # noqa: E402
+import zoo  # noqa: E402
+from at_amd.lattice import Lattice, Particle, load_mat  # noqa: E402
+from at_amd.workloads import fodo_ring, gaussian_beam, large_ring  # noqa: E402
+
+REFS3 = np.array([0, 1, 2], np.uint32)
+
+
+def make_zoo():
+    out = {}
+    names = []
+    for name, elem, energy in zoo.cases():
+        names.append(name)
+        r0 = zoo.beam(24, seed=101, scale=0.3)
+        r = r0.copy(order="F")
+        po.elempass(elem, r, energy=energy, impl="ref")
+        out[f"{name}/ep_in"] = r0
+        out[f"{name}/ep_out"] = r
+        lat = Lattice([elem, elem.copy()], energy=energy, particle=Particle("electron"))
+        r0 = zoo.beam(24, seed=202, scale=0.5)
+        r = r0.copy(order="F")
+        rout, lm = po.atpass(lat, r, 3, refpts=REFS3, turn=3, losses=True, impl="ref")
+        out[f"{name}/lt_in"] = r0
+        out[f"{name}/lt_out"] = r
+        out[f"{name}/lt_rout"] = rout
+        for k, v in lm.items():
+            out[f"{name}/lt_{k}"] = np.asarray(v)
+    out["names"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, "zoo.npz"), **out)
+
+
+def make_fodo():
+    ring = fodo_ring()
+    r0 = gaussian_beam(64, seed=3)
+    r = r0.copy(order="F")
+    refs = np.array([0, 5, 160, 320], np.uint32)
+    rout = po.atpass(ring, r, 10, refpts=refs, impl="ref")
+    np.savez_compressed(os.path.join(HERE, "fodo.npz"), rin=r0, rout_final=r, rout=rout,
+                        refpts=refs, nturns=10)
+
+
+def make_large_ring():
+    ring = large_ring()
+    r0 = gaussian_beam(48, seed=4, scale=10.0)
+    r = r0.copy(order="F")
+    rout, lm = po.atpass(ring, r, 3, refpts=np.array([0, len(ring)], np.uint32), losses=True,
+                         impl="ref")
+    np.savez_compressed(os.path.join(HERE, "large_ring.npz"), rin=r0, rout_final=r, rout=rout,
+                        nturns=3, **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
+
+
+def make_hmba(mat):
+    ring = load_mat(mat)
+    out = {}
+    r = np.asfortranarray(np.array([0, 1e-6, 0, 0, 0, 0], dtype=float).reshape(6, 1))
+    po.atpass(ring, r, 1, start_elem=100, impl="ref")
+    out["start100"] = r
+    r = np.asfortranarray(np.array([0, 1e-6, 0, 0, 0, 0], dtype=float).reshape(6, 1))
+    po.atpass(ring, r, 1, start_elem=100, end_elem=110, impl="ref")
+    out["start100_end110"] = r
+    r0 = gaussian_beam(8, seed=5, scale=5.0)
+    r = r0.copy(order="F")
+    refs = np.array([0, 30, 61, len(ring)], np.uint32)
+    rout, lm = po.atpass(ring, r, 5, refpts=refs, losses=True, impl="ref")
+    out.update(rin=r0, rout_final=r, rout=rout, refpts=refs,
+               **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
+    np.savez_compressed(os.path.join(HERE, "hmba.npz"), **out)
+
+
+if __name__ == "__main__":
+    if not po.available("ref"):
+        sys.exit("oracle/_ref is not built: run `make -C oracle ref` with /root/reference present")
+    make_zoo()
+    make_fodo()
+    make_large_ring()
+    make_hmba(os.path.join(HERE, "hmba.mat"))
+    print("golden fixtures written to", HERE)

@@ -1,0 +1,117 @@
+"""CPU: the C ABI library loads, exports every declared symbol, and its host
+logic (element registry, attribute validation, partial-turn checks) follows
+the reference error semantics, all without a GPU (no compute calls)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from at_amd import engine
+from at_amd.lattice import Dipole, Drift, Element, Marker, Quadrupole, RFCavity
+from at_amd.tracking import atpass, get_uint32_index, End
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "atx.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(atx_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load_library()
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(engine.EXPORTS)
+
+
+def test_unknown_pass_method_is_runtime_error():
+    with pytest.raises(RuntimeError, match="Not implemented PassMethod: UnknownPass"):
+        engine.DeviceLattice([Drift("d", 1.0, PassMethod="UnknownPass")])
+
+
+def test_missing_required_attribute_is_attribute_error():
+    d = Drift("d", 1.0)
+    del d.Length
+    lat = engine.DeviceLattice([d])
+    with pytest.raises(AttributeError, match="Length"):
+        lat.check(0.0, 0.0)
+    q = Quadrupole("q", 0.5, 1.0)
+    del q.NumIntSteps
+    with pytest.raises(AttributeError, match="NumIntSteps"):
+        engine.DeviceLattice([q]).check(0.0, 0.0)
+    b = Dipole("b", 1.0, 0.1)
+    del b.ExitAngle
+    with pytest.raises(AttributeError, match="ExitAngle"):
+        engine.DeviceLattice([b]).check(0.0, 0.0)
+
+
+def test_radiation_and_rf_need_energy():
+    q = Quadrupole("q", 0.5, 1.0, PassMethod="StrMPoleSymplectic4RadPass")
+    with pytest.raises(ValueError, match="Energy must be positive"):
+        engine.DeviceLattice([q]).check(0.0, 0.0)
+    engine.DeviceLattice([q]).check(6e9, 0.0)
+    rf = RFCavity("rf", 0.0, 1e6, 3.5e8, 31, 6e9)
+    with pytest.raises(ValueError):
+        engine.DeviceLattice([rf]).check(0.0, 0.0)
+
+
+def test_array_size_validation():
+    d = Drift("d", 1.0)
+    object.__setattr__(d, "T1", np.zeros(3))
+    with pytest.raises(ValueError, match="T1"):
+        engine.DeviceLattice([d]).check(0.0, 0.0)
+
+
+def test_missing_pass_method_attribute_error():
+    m = Marker("m")
+    del m.PassMethod
+    with pytest.raises(AttributeError):
+        engine.DeviceLattice([m])
+
+
+def test_partial_turn_errors_before_any_device_work():
+    lat = engine.DeviceLattice([Drift("d", 1.0)] * 20)
+    rin = np.asfortranarray(np.zeros((6, 1)))
+    for kw, nt in ((dict(start_elem=30), 1), (dict(end_elem=30), 1),
+                   (dict(start_elem=10, end_elem=5), 1), (dict(end_elem=10), 2)):
+        with pytest.raises(ValueError):
+            lat.track(rin, nt, None, **kw)
+
+
+def test_atpass_argument_errors():
+    # pyat/test/test_atpass.py:8-35
+    rin = np.asfortranarray(np.zeros((6, 1)))
+    with pytest.raises(TypeError):
+        atpass(1, rin, 1)
+    with pytest.raises(TypeError):
+        atpass([], 1, 1)
+    with pytest.raises(TypeError):
+        atpass([], rin, "a")
+    with pytest.raises(ValueError):
+        atpass([], np.zeros((7, 1)), 1)
+    with pytest.raises(ValueError):
+        atpass([], np.zeros((2, 6)), 1)
+    with pytest.raises(ValueError):
+        atpass([], np.asfortranarray(np.zeros((2, 6))).T, 1)
+
+
+def test_refpts_normalisation():
+    lat = [Drift("d", 1.0)] * 5
+    np.testing.assert_array_equal(get_uint32_index(lat, End), [5])
+    np.testing.assert_array_equal(get_uint32_index(lat, None), [])
+    np.testing.assert_array_equal(get_uint32_index(lat, [3, 0, -1, 3]), [0, 3, 5])
+    np.testing.assert_array_equal(get_uint32_index(lat, [True, False, True, False, False, True]),
+                                  [0, 2, 5])
+    assert get_uint32_index(lat, 2).dtype == np.uint32
+
+
+def test_device_count_without_gpu_is_zero_or_more():
+    assert engine.device_count() >= 0

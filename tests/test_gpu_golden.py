@@ -1,0 +1,219 @@
+"""GPU: the engine against the reference's golden vectors and known answers,
+through the pyat-style API (lattice_track / element_track with
+use_mp=MPMode.GPU) and through the C ABI.
+
+Bars: strict build bit-exact where no device transcendental enters, else
+|diff| <= 1e-15; fast build |diff| <= 1e-12 (one pass) / 1e-11 (3 turns);
+loss flags, turns and elements identical in both builds.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import zoo
+from at_amd import engine
+from at_amd.lattice import Aperture, Corrector, Dipole, Drift, Lattice, Marker, Particle, \
+    Quadrupole, RFCavity, load_mat
+from at_amd.tracking import MPMode, atpass, element_track, gpu_info, internal_lpass, \
+    lattice_track
+from at_amd.workloads import fodo_ring, large_ring
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TRANSCENDENTAL = {"rf_thin", "rf_thick_lag", "bend_fringe_m1", "bend_fringe_m2",
+                  "bend_fringe_m3", "bend_rad_fringe_m1", "bend_rad_fringe_m2",
+                  "bend_rad_fringe_m3"}
+CASES = {c[0]: c for c in zoo.cases()}
+REFS3 = np.array([0, 1, 2], np.uint32)
+
+
+def _cmp(a, b, exact, tol, msg=""):
+    if exact:
+        np.testing.assert_array_equal(a, b, err_msg=msg)
+    else:
+        np.testing.assert_allclose(a, b, rtol=0, atol=tol, equal_nan=True, err_msg=msg)
+
+
+@pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_zoo(name, strict):
+    g = np.load(os.path.join(GOLD, "zoo.npz"))
+    _, elem, energy = CASES[name]
+    exact = strict and name not in TRANSCENDENTAL
+    dl = engine.DeviceLattice([elem])
+    rin = g[f"{name}/ep_in"]
+    r = rin.copy(order="F")
+    dl.elempass(0, r, energy=energy, strict=strict)
+    fin = np.all(np.isfinite(rin), axis=0)
+    _cmp(r[:, fin], g[f"{name}/ep_out"][:, fin], exact, 1e-15 if strict else 1e-12, name)
+    lat = Lattice([elem, elem.copy()], energy=energy, particle=Particle("electron"))
+    dl = engine.DeviceLattice(lat)
+    r = g[f"{name}/lt_in"].copy(order="F")
+    rout, lm = dl.track(r, 3, REFS3, turn=3, energy=energy,
+                        rest_energy=lat.particle.rest_energy, losses=True, strict=strict)
+    for k in ("islost", "turn", "elem"):
+        np.testing.assert_array_equal(np.asarray(lm[k]), g[f"{name}/lt_{k}"], err_msg=name + k)
+    tol = 1e-15 if strict else 1e-11
+    _cmp(lm["coord"], g[f"{name}/lt_coord"], exact, tol, name)
+    _cmp(r, g[f"{name}/lt_out"], exact, tol, name)
+    _cmp(rout, g[f"{name}/lt_rout"], exact, tol, name)
+
+
+@pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
+def test_golden_fodo(strict):
+    g = np.load(os.path.join(GOLD, "fodo.npz"))
+    ring = fodo_ring()
+    r = g["rin"].copy(order="F")
+    rout, _, _ = lattice_track(ring, r, int(g["nturns"]), refpts=g["refpts"], in_place=True,
+                               use_mp=MPMode.GPU, strict=strict)
+    _cmp(r, g["rout_final"], strict, 1e-12)
+    _cmp(rout, g["rout"], strict, 1e-12)
+
+
+@pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
+def test_golden_large_ring(strict):
+    """Radiation, RF (device sin: not bit-exact) and apertures; loss indices
+    must be identical."""
+    g = np.load(os.path.join(GOLD, "large_ring.npz"))
+    ring = large_ring()
+    r = g["rin"].copy(order="F")
+    rout, _, td = lattice_track(ring, r, int(g["nturns"]),
+                                refpts=np.array([0, len(ring)], np.uint32), in_place=True,
+                                losses=True, use_mp=MPMode.GPU, strict=strict)
+    lm = td["loss_map"]
+    np.testing.assert_array_equal(lm["islost"], g["loss_islost"])
+    np.testing.assert_array_equal(lm["turn"], g["loss_turn"])
+    np.testing.assert_array_equal(lm["elem"], g["loss_elem"])
+    surv = ~g["loss_islost"]
+    np.testing.assert_allclose(r[:, surv], g["rout_final"][:, surv], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rout, g["rout"], rtol=0, atol=1e-12, equal_nan=True)
+
+
+def test_hmba_known_answers_and_golden():
+    """pyat/test/test_atpass.py:157-171 through lattice_track on the GPU."""
+    g = np.load(os.path.join(GOLD, "hmba.npz"))
+    ring = load_mat(os.path.join(GOLD, "hmba.mat")).radiation_off(copy=True)
+    r = np.asfortranarray(np.array([0, 1e-6, 0, 0, 0, 0.0]).reshape(6, 1))
+    rout, _, _ = lattice_track(ring, r, 1, refpts=None, in_place=True, start_elem=100,
+                               use_mp=MPMode.GPU, strict=True)
+    assert rout.shape == (6, 1, 0, 1)
+    np.testing.assert_array_equal(r, g["start100"])
+    np.testing.assert_allclose(r[:, 0], [3.2e-6, -1.1e-6, 0, 0, 0, 8.4e-8], atol=1e-7)
+    r = np.asfortranarray(np.array([0, 1e-6, 0, 0, 0, 0.0]).reshape(6, 1))
+    lattice_track(ring, r, 1, refpts=None, in_place=True, start_elem=100, end_elem=110,
+                  use_mp=MPMode.GPU, strict=True)
+    np.testing.assert_array_equal(r, g["start100_end110"])
+    np.testing.assert_allclose(r[:, 0], [2.6e-6, 1.34e-6, 0, 0, 0, 2.4e-8], atol=1e-7)
+    r = g["rin"].copy(order="F")
+    rout, _, td = lattice_track(ring, r, 5, refpts=g["refpts"], in_place=True, losses=True,
+                                use_mp=MPMode.GPU, strict=True)
+    np.testing.assert_array_equal(rout, g["rout"])
+    np.testing.assert_array_equal(r, g["rout_final"])
+    np.testing.assert_array_equal(td["loss_map"]["islost"], g["loss_islost"])
+
+
+# --- pyat/test known answers through element_track / atpass -------------------
+
+def _r(*v):
+    return np.asfortranarray(np.array(v, dtype=float).reshape(6, -1))
+
+
+def test_element_known_answers():
+    r = _r(0, 1e-6, 0, -2e-6, 0, 0)
+    element_track(Drift("d", 1.0), r, in_place=True)
+    np.testing.assert_array_equal(r[:, 0], [1e-6, 1e-6, -2e-6, -2e-6, 0, 2.5e-12])
+    r = _r(1e-6, 0, 0, 0, 0, 0)
+    element_track(Quadrupole("quad", 0.4, k=1), r, in_place=True)
+    np.testing.assert_allclose(
+        r[:, 0], np.array([0.9210610203854122, -0.3894182419439, 0, 0, 0, 0.0000000103303797478]) * 1e-6)
+    r = _r(0, 0, 0, 0, 1e-6, 1e-6)
+    element_track(RFCavity("rfcavity", 0.0, 187500, 3.5237e8, 31, 6.0e9), r, in_place=True,
+                  energy=6.0e9)
+    np.testing.assert_allclose(r[:, 0], [0, 0, 0, 0, 9.99769215e-07, 1.0e-6], atol=1e-12)
+    r = _r(1e-6, 0, 0, 0, 0, 0)
+    element_track(Dipole("dipole", 1.0, 0.1, EntranceAngle=0.05, ExitAngle=0.05), r,
+                  in_place=True)
+    np.testing.assert_almost_equal(r[:, 0], [1e-6, 0, 0, 0, 0, 1e-7])
+    r = _r(1e-6, 0, 0, 0, 0, 0)
+    element_track(Corrector("c", 0.0, np.array([0.9, 0.5])), r, in_place=True)
+    np.testing.assert_array_equal(r[:, 0], [1e-6, 0.9, 0, 0.5, 0, 0])
+    r = np.asfortranarray(np.random.default_rng(0).random((6, 4)))
+    r0 = r.copy()
+    element_track(Marker("m"), r, in_place=True)
+    np.testing.assert_array_equal(r, r0)
+
+
+def test_aperture_outside_limits_lattice_track():
+    # pyat/test/test_basic_elements.py:429-442
+    a = Aperture("aperture", [-1e-3, 1e-3, -1e-4, 1e-4])
+    r = _r(1e-2, 0, -1e-2, 0, 0, 0)
+    lattice_track([a], r, in_place=True, use_mp=MPMode.GPU)
+    assert np.isnan(r[0, 0]) and r[2, 0] == 0.0
+
+
+def test_atpass_two_particles_two_turns_and_1d():
+    # pyat/test/test_atpass.py:107-155
+    rin = np.asfortranarray(np.zeros((6, 2)))
+    rin[1, 0] = 1e-6
+    rin[3, 0] = -2e-6
+    rout = atpass([Drift("drift", 1.0)], rin, 2, refpts=np.array([1], np.uint32))
+    np.testing.assert_array_equal(rout[:, 0, 0, 0], [1e-6, 1e-6, -2e-6, -2e-6, 0, 2.5e-12])
+    np.testing.assert_array_equal(rout[:, 0, 0, 1], [2e-6, 1e-6, -4e-6, -2e-6, 0, 5e-12])
+    np.testing.assert_array_equal(rout[:, 1, 0, :], 0.0)
+    rin = np.zeros(6)
+    rin[1] = 1e-6
+    rout = atpass([Drift("drift", 1.0)], rin, 1)
+    assert rout.shape == (6, 1, 0, 1)
+    np.testing.assert_array_equal(rin, [1e-6, 1e-6, 0, 0, 0, 5e-13])
+
+
+def test_transposed_c_array_same_as_fortran():
+    # pyat/test/test_atpass.py:43-63
+    rin_f = np.array(np.arange(12).reshape(6, 2), order="F") * 1e-5
+    rin_c = np.copy(rin_f.T, order="C")
+    lat = [Drift("drift", 1.0)]
+    out_f = atpass(lat, rin_f, 1)
+    out_c = atpass(lat, rin_c.T, 1)
+    np.testing.assert_equal(out_c, out_f)
+
+
+@pytest.mark.parametrize("reuse", (True, False))
+def test_reuse_attributes(reuse):
+    # pyat/test/test_atpass.py:86-104
+    lat = [Drift("drift", 1.0)]
+    rin = np.asfortranarray(np.zeros((6, 1)))
+    rin[0, 0] = 1e-6
+    rin[1, 0] = 1e-6
+    rin_copy = np.copy(rin)
+    atpass(lat, rin, 2)
+    atpass(lat, rin_copy, 1)
+    lat[0].Length = 2
+    atpass(lat, rin_copy, 1, reuse=reuse)
+    if reuse:
+        np.testing.assert_equal(rin, rin_copy)
+    else:
+        with pytest.raises(AssertionError):
+            np.testing.assert_equal(rin, rin_copy)
+
+
+def test_keep_counter_continues_turns():
+    """The turn counter carries over (at.c:428-431,656): 2+3 turns with
+    keep_counter equal 5 turns in one call, RF included."""
+    ring = large_ring(ncells=2)
+    r0 = np.asfortranarray(np.random.default_rng(3).normal(0, 1e-4, (6, 32)))
+    a = r0.copy(order="F")
+    internal_lpass(ring, a, 5, refpts=None, strict=True)
+    b = r0.copy(order="F")
+    internal_lpass(ring, b, 2, refpts=None, strict=True, turn=0)
+    internal_lpass(ring, b, 3, refpts=None, strict=True, keep_counter=True, keep_lattice=True)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_gpu_info():
+    info = gpu_info()
+    assert len(info) >= 1
+    name, arch, cus, platform = info[0]
+    assert arch.startswith("gfx") and cus > 0 and platform.startswith("HIP")
