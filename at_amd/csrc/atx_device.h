@@ -89,10 +89,14 @@ struct DevElem {
   int32_t t1, r1;     // pool offsets (valid when the flag is set)
   int32_t r2, t2;
   int32_t rap, eap;   // RApertures (4) / Limits (4); EApertures (2)
-  int32_t edge_in, edge_out;   // EdgeMode
+  int32_t edge;       // EdgeMode at entrance | (EdgeMode at exit << 8)
+  int32_t ops;        // pool offset of the slice op table (integrators other than 4), -1: none
   int32_t fringe;     // QuadFringe at entrance | (QuadFringe at exit << 8)
   int32_t fq;         // pool offset of the elegant fringe matrices: in-P, in-M, out-M, out-P (8 each)
 };
+
+// Slice op table of a non-default integrator: pool[ops] = number of ops,
+// then (code, length) pairs with code 0 = drift of that length, 1 = kick.
 static_assert(sizeof(DevElem) == 64, "DevElem must stay 64 bytes (dword fields: scalar loads)");
 
 // One tracked lattice instance inside a launch. A single-lattice call has one

@@ -86,6 +86,38 @@ struct Compiled {
   double T0 = 0.0;
 };
 
+// Drift (c) and kick (d) coefficients per slice of the integrator types of
+// atgpu/SymplecticIntegrator.cpp:32-86 (published coefficient sets: Euler;
+// McLachlan & Atela 1991 2nd and 4th order; Ruth 1983 3rd order; Forest &
+// Ruth 1990 4th order; Yoshida 1990 6th order, solution A).
+struct Scheme {
+  int n;
+  double c[8], d[8];
+};
+
+bool integrator_scheme(int type, Scheme* s) {
+  const double r2 = 1.0 / std::sqrt(2.0);
+  switch (type) {
+    case 1: *s = {1, {1.0}, {1.0}}; return true;
+    case 2: *s = {2, {1.0 - r2, r2}, {1.0 - r2, r2}}; return true;
+    case 3: *s = {3, {1.0, -2.0 / 3.0, 2.0 / 3.0}, {-1.0 / 24.0, 3.0 / 4.0, 7.0 / 24.0}}; return true;
+    case 4: *s = {4, {kDrift1, kDrift2, kDrift2, kDrift1}, {kKick1, kKick2, kKick1, 0.0}}; return true;
+    case 5:
+      *s = {4, {0.1288461583653843, 0.4415830236164665, -0.0857820194129737, 0.5153528374311229},
+               {0.3340036032863214, 0.7563200005156683, -0.2248198030794208, 0.1344961992774311}};
+      return true;
+    case 6:  // the 16-digit values atgpu tabulates from Yoshida's w1..w3
+      *s = {8,
+            {0.3922568052387800, 0.5100434119184585, -0.4710533854097566, 0.0687531682525181,
+             0.0687531682525181, -0.4710533854097566, 0.5100434119184585, 0.3922568052387800},
+            {0.7845136104775600, 0.2355732133593570, -1.1776799841788701, 1.3151863206839063,
+             -1.1776799841788701, 0.2355732133593570, 0.7845136104775600, 0.0}};
+      return true;
+    default:
+      return false;
+  }
+}
+
 }  // namespace
 }  // namespace atx
 
@@ -100,6 +132,7 @@ struct atx_lattice {
   double* d_pool = nullptr;
   size_t d_elems_n = 0, d_pool_n = 0;
   int64_t turn_counter = 0;
+  int integrator = 4;
   std::mutex mu;
 };
 
@@ -225,12 +258,13 @@ int32_t edge_mode(bool rad, int method, double irho, double angle, double fint, 
   return rad ? E_RTAN1 : E_TAN1;
 }
 
-int compile_elem(const HostElem& e, int idx, double energy, double rest, double T0,
+int compile_elem(const HostElem& e, int idx, double energy, double rest, double T0, int integrator,
                  Compiled& out) {
   Reader rd{e, idx};
   DevElem h;
   std::memset(&h, 0, sizeof(h));
   h.kind = e.kind;
+  h.ops = -1;
   std::vector<double>& pool = out.pool;
   double sc[S_NSCALAR];
   for (double& v : sc) v = 0.0;
@@ -344,6 +378,19 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       sc[S_L2] = sl * kDrift2;
       sc[S_K1] = sl * kKick1;
       sc[S_K2] = sl * kKick2;
+      if (integrator != 4) {
+        // op table: drifts and kicks of SL*c_i / SL*d_i, zero steps omitted
+        // as atgpu's generated slice loop omits them (SymplecticIntegrator.cpp:140-171)
+        Scheme sch;
+        integrator_scheme(integrator, &sch);
+        std::vector<double> ops(1, 0.0);
+        for (int i = 0; i < sch.n; i++) {
+          if (sch.c[i] != 0.0) { ops.push_back(0.0); ops.push_back(sl * sch.c[i]); }
+          if (sch.d[i] != 0.0) { ops.push_back(1.0); ops.push_back(sl * sch.d[i]); }
+        }
+        ops[0] = (double)((ops.size() - 1) / 2);
+        h.ops = push(pool, ops.data(), ops.size());
+      }
       const double b2 = (*pb)[1];
       sc[S_B2] = b2;
       const long fqi = rd.opt_lng("FringeQuadEntrance", 0);
@@ -370,10 +417,11 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
         const double gap = rd.opt_dbl("FullGap", 0.0);
         const double f1 = rd.opt_dbl("FringeInt1", 0.0);
         const double f2 = rd.opt_dbl("FringeInt2", 0.0);
-        h.edge_in = edge_mode(rad, mi, irho, ein, f1, gap, &sc[S_FX_IN], &sc[S_FY_IN], &sc[S_CORR_IN]);
+        const int32_t ein_mode = edge_mode(rad, mi, irho, ein, f1, gap, &sc[S_FX_IN], &sc[S_FY_IN], &sc[S_CORR_IN]);
         sc[S_EDGE_IN] = ein;
-        h.edge_out = edge_mode(rad, mo, irho, eout, f2, gap, &sc[S_FX_OUT], &sc[S_FY_OUT], &sc[S_CORR_OUT]);
+        const int32_t eout_mode = edge_mode(rad, mo, irho, eout, f2, gap, &sc[S_FX_OUT], &sc[S_FY_OUT], &sc[S_CORR_OUT]);
         sc[S_EDGE_OUT] = eout;
+        h.edge = ein_mode | (eout_mode << 8);
       }
       if (rad) {
         if (!(energy > 0.0)) {
@@ -402,7 +450,7 @@ int compile_lattice(const atx_lattice* lat, double energy, double rest, bool ele
   out.elems.clear();
   out.pool.clear();
   for (size_t i = 0; i < lat->elems.size(); i++) {
-    int rc = compile_elem(lat->elems[i], (int)i, energy, rest, out.T0, out);
+    int rc = compile_elem(lat->elems[i], (int)i, energy, rest, out.T0, lat->integrator, out);
     if (rc) return rc;
   }
   if (out.pool.empty()) out.pool.push_back(0.0);
@@ -465,20 +513,62 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
 
 constexpr int kBlock = 256;
 
+// Launch one kernel over several instances: instances get consecutive
+// workgroup ranges; one stream-ordered scratch block carries the instance
+// table and every instance's refpt slot table.
+int launch_instances(std::vector<Instance>& insts, const std::vector<std::vector<int32_t>>& slots,
+                     int block, hipStream_t stream, bool strict) {
+  if (block <= 0) block = kBlock;
+  if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
+  int nblocks = 0;
+  for (auto& in : insts) {
+    in.block_begin = nblocks;
+    nblocks += (int)((in.npart + block - 1) / block);
+  }
+  if (nblocks == 0) return ATX_OK;
+  size_t bytes = insts.size() * sizeof(Instance);
+  std::vector<size_t> soff(insts.size());
+  for (size_t i = 0; i < insts.size(); i++) {
+    soff[i] = bytes;
+    bytes += slots[i].size() * sizeof(int32_t);
+  }
+  char* d_scratch = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d_scratch, bytes, stream));
+  std::vector<char> host(bytes);
+  for (size_t i = 0; i < insts.size(); i++) {
+    insts[i].slots = (const int32_t*)(d_scratch + soff[i]);
+    std::memcpy(host.data() + soff[i], slots[i].data(), slots[i].size() * sizeof(int32_t));
+  }
+  std::memcpy(host.data(), insts.data(), insts.size() * sizeof(Instance));
+  hipError_t ce = hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream);
+  hipError_t le = hipSuccess;
+  if (ce == hipSuccess)
+    le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block, stream)
+                : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block, stream);
+  hipError_t fe = hipFreeAsync(d_scratch, stream);
+  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  // hipMemcpyAsync from pageable memory returns once the source is staged,
+  // so `host` may go out of scope here.
+  return ATX_OK;
+}
+
 // Device-side core of atx_track: all pointers are device pointers.
 int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
                  int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
                  double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
                  int losses, bool elempass, hipStream_t stream, bool strict) {
   const int nelem = (int)lat->elems.size();
-  std::vector<int32_t> slots(2 * (size_t)(nelem + 1));
-  const int n_first = turn_slots(refpts, (int)nref, start, end, nelem, slots.data());
-  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, slots.data() + nelem + 1);
+  std::vector<std::vector<int32_t>> slots(1, std::vector<int32_t>(2 * (size_t)(nelem + 1)));
+  const int n_first = turn_slots(refpts, (int)nref, start, end, nelem, slots[0].data());
+  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, slots[0].data() + nelem + 1);
   if ((n_first > 0 || n_std > 0) && !d_rout) return fail(ATX_E_VALUE, "r_out is NULL but refpts request copies");
   if (losses && (!d_lf || !d_lt || !d_le || !d_lc)) return fail(ATX_E_VALUE, "loss arrays are NULL");
   if (np == 0 || nturns <= 0) return ATX_OK;
 
-  Instance inst;
+  std::vector<Instance> insts(1);
+  Instance& inst = insts[0];
   std::memset(&inst, 0, sizeof(inst));
   inst.elems = lat->d_elems;
   inst.pool = lat->d_pool;
@@ -501,25 +591,7 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   inst.block_begin = 0;
   inst.losses = losses ? 1 : 0;
   inst.elempass = elempass ? 1 : 0;
-
-  // One stream-ordered scratch block: [Instance][slots]
-  const size_t bytes = sizeof(Instance) + slots.size() * sizeof(int32_t);
-  std::vector<char> host(bytes);
-  char* d_scratch = nullptr;
-  HIPCHK(hipMallocAsync((void**)&d_scratch, bytes, stream));
-  inst.slots = (const int32_t*)(d_scratch + sizeof(Instance));
-  std::memcpy(host.data(), &inst, sizeof(Instance));
-  std::memcpy(host.data() + sizeof(Instance), slots.data(), slots.size() * sizeof(int32_t));
-  HIPCHK(hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream));
-  const int nblocks = (int)((np + kBlock - 1) / kBlock);
-  hipError_t le = strict ? atx_launch_track_strict((const Instance*)d_scratch, 1, nblocks, kBlock, stream)
-                         : atx_launch_track_fast((const Instance*)d_scratch, 1, nblocks, kBlock, stream);
-  hipError_t fe = hipFreeAsync(d_scratch, stream);
-  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
-  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
-  // hipMemcpyAsync from pageable memory returns once the source is staged,
-  // so `host` may go out of scope here.
-  return ATX_OK;
+  return launch_instances(insts, slots, kBlock, stream, strict);
 }
 
 int check_partial(int nelem, int nturns, int& start, int& end) {
@@ -611,6 +683,19 @@ int atx_element_set_array(atx_lattice* lat, const char* name, const double* data
 }
 
 int atx_lattice_num_elements(const atx_lattice* lat) { return lat ? (int)lat->elems.size() : 0; }
+
+int atx_lattice_set_integrator(atx_lattice* lat, int32_t type) {
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  Scheme s;
+  if (!integrator_scheme(type, &s))
+    return fail(ATX_E_RUNTIME, "SymplecticIntegrator, type " + std::to_string(type) + " not supported");
+  std::lock_guard<std::mutex> g(lat->mu);
+  if (lat->integrator != type) {
+    lat->integrator = type;
+    lat->dev_valid = false;  // recompiled, as atgpu recreates its lattice (PyInterface.cpp:259-266)
+  }
+  return ATX_OK;
+}
 
 int atx_lattice_check(atx_lattice* lat, double energy, double rest_energy) {
   if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
@@ -719,6 +804,56 @@ int atx_track(atx_lattice* lat, double* r_in, uint32_t np, int32_t nturns, const
   }
   lat->turn_counter = (int64_t)turn0 + nturns;  // at.c:652,656
   return ATX_OK;
+}
+
+int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns, int32_t turn,
+                    double energy, double rest_energy, int32_t losses, int32_t block_size,
+                    void* stream_, uint32_t flags) {
+  g_err.clear();
+  if (nitems < 0 || (nitems > 0 && !items)) return fail(ATX_E_VALUE, "bad item list");
+  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool strict = (flags & ATX_STRICT) != 0;
+  const bool soa = (flags & ATX_SOA) != 0;
+  std::vector<Instance> insts;
+  std::vector<std::vector<int32_t>> slots;
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (int i = 0; i < nitems; i++) {
+    const atx_batch_item& it = items[i];
+    if (!it.lattice) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL lattice");
+    if (it.num_particles > 0 && !it.r) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL r");
+    if (losses && it.num_particles > 0 && (!it.lost_flag || !it.lost_turn || !it.lost_elem || !it.lost_coord))
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": loss arrays are NULL");
+    bool seen = false;
+    for (int j = 0; j < i; j++) seen |= (items[j].lattice == it.lattice);
+    if (!seen) locks.emplace_back(it.lattice->mu);
+    int rc = ensure_device_lattice(it.lattice, energy, rest_energy, false);
+    if (rc) return fail(rc, "item " + std::to_string(i) + ": " + g_err);
+    if (it.num_particles == 0) continue;
+    const int nelem = (int)it.lattice->elems.size();
+    Instance in;
+    std::memset(&in, 0, sizeof(in));
+    in.elems = it.lattice->d_elems;
+    in.pool = it.lattice->d_pool;
+    in.r = it.r;
+    in.lost_flag = it.lost_flag;
+    in.lost_turn = it.lost_turn;
+    in.lost_elem = it.lost_elem;
+    in.lost_coord = it.lost_coord;
+    in.pstride = soa ? 1 : 6;
+    in.cstride = soa ? (int64_t)it.num_particles : 1;
+    in.nelem = nelem;
+    in.npart = (int32_t)it.num_particles;
+    in.nturns = nturns;
+    in.turn0 = turn;
+    in.start_elem = 0;
+    in.end_elem = nelem;
+    in.losses = losses ? 1 : 0;
+    insts.push_back(in);
+    slots.emplace_back(2 * (size_t)(nelem + 1), -1);
+  }
+  if (nturns == 0 || insts.empty()) return ATX_OK;
+  return launch_instances(insts, slots, block_size, stream, strict);
 }
 
 int atx_elempass(atx_lattice* lat, int32_t index, double* r_in, uint32_t np, double energy,

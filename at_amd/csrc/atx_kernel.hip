@@ -394,6 +394,37 @@ __device__ __forceinline__ void quad_fringe_out(R6& r, cDevElem& h, cdouble* poo
   r = quad_fringe_v(r, (h.fringe >> 8) & 0xff, pool + h.fq + 16, b2, false);
 }
 
+// Slices of a non-default integrator (atgpu/SymplecticIntegrator.cpp types
+// 1-3, 5, 6): the host-built op table lists the drifts and kicks of one
+// slice; kept out of line, generic in the multipole order.
+template <bool BEND, bool RAD>
+__device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdouble* op, int nslice,
+                                      double pn0, double irho, double radc) {
+  const int nop = (int)op[0];
+#if ATX_STRICT
+  const double h2 = 0.0;
+#else
+  const double h2 = 0.5 / (1.0 + r.dp);
+#endif
+  for (int m = 0; m < nslice; m++) {
+    for (int j = 0; j < nop; j++) {
+      const double len = op[2 + 2 * j];
+      if (op[1 + 2 * j] == 0.0) {
+        if (RAD) rad_drift(r, len); else fastdrift(r, len * pn0, h2);
+      } else {
+        if (RAD) {
+          const double pn = 1.0 / (1.0 + r.dp);
+          if (BEND) rad_bnd_kick<-1>(r, a, b, n, len, irho, len * irho, radc, pn);
+          else rad_str_kick<-1>(r, a, b, n, len, radc, pn);
+        } else {
+          if (BEND) bnd_kick<-1>(r, a, b, n, len, irho, len * irho); else str_kick<-1>(r, a, b, n, len);
+        }
+      }
+    }
+  }
+  return r;
+}
+
 // StrMPoleSymplectic4Pass.c:57-104 and BndMPoleSymplectic4Pass.c:75-122.
 template <bool BEND, int N>
 __device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
@@ -408,26 +439,30 @@ __device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
   entry_xform(r, h, pool);
   apertures(r, h, pool);
   const double irho = s[S_IRHO];
-  if (BEND) edge(r, h.edge_in, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
+  if (BEND) edge(r, h.edge & 0xff, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
   if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
-  const double k1 = s[S_K1], k2 = s[S_K2];
-  const double ki1 = k1 * irho, ki2 = k2 * irho;
+  if (h.ops >= 0) {
+    r = slices_ops<BEND, false>(r, a, b, n, pool + h.ops, h.nslice, pn, irho, 0.0);
+  } else {
+    const double k1 = s[S_K1], k2 = s[S_K2];
+    const double ki1 = k1 * irho, ki2 = k2 * irho;
 #if ATX_STRICT
-  const double h2 = 0.0;
+    const double h2 = 0.0;
 #else
-  const double h2 = 0.5 / (1.0 + r.dp);  // delta is constant through the slices
+    const double h2 = 0.5 / (1.0 + r.dp);  // delta is constant through the slices
 #endif
-  for (int m = 0; m < h.nslice; m++) {
-    fastdrift(r, nl1, h2);
-    if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
-    fastdrift(r, nl2, h2);
-    if (BEND) bnd_kick<N>(r, a, b, n, k2, irho, ki2); else str_kick<N>(r, a, b, n, k2);
-    fastdrift(r, nl2, h2);
-    if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
-    fastdrift(r, nl1, h2);
+    for (int m = 0; m < h.nslice; m++) {
+      fastdrift(r, nl1, h2);
+      if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+      fastdrift(r, nl2, h2);
+      if (BEND) bnd_kick<N>(r, a, b, n, k2, irho, ki2); else str_kick<N>(r, a, b, n, k2);
+      fastdrift(r, nl2, h2);
+      if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+      fastdrift(r, nl1, h2);
+    }
   }
   if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
-  if (BEND) edge(r, h.edge_out, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
+  if (BEND) edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
   apertures(r, h, pool);
   exit_xform(r, h, pool);
   if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
@@ -444,25 +479,29 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
   entry_xform(r, h, pool);
   apertures(r, h, pool);
   const double irho = s[S_IRHO];
-  if (BEND) edge(r, h.edge_in, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
+  if (BEND) edge(r, h.edge & 0xff, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
   if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
-  const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
-  const double ki1 = k1 * irho, ki2 = k2 * irho;
   const double radc = s[S_RADC];
-  for (int m = 0; m < h.nslice; m++) {
-    double pn = rad_drift(r, l1);
-    if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-    else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
-    pn = rad_drift(r, l2);
-    if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
-    else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
-    pn = rad_drift(r, l2);
-    if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-    else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
-    rad_drift(r, l1);
+  if (h.ops >= 0) {
+    r = slices_ops<BEND, true>(r, a, b, n, pool + h.ops, h.nslice, 0.0, irho, radc);
+  } else {
+    const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
+    const double ki1 = k1 * irho, ki2 = k2 * irho;
+    for (int m = 0; m < h.nslice; m++) {
+      double pn = rad_drift(r, l1);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      pn = rad_drift(r, l2);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
+      pn = rad_drift(r, l2);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      rad_drift(r, l1);
+    }
   }
   if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
-  if (BEND) edge(r, h.edge_out, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
+  if (BEND) edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
   apertures(r, h, pool);
   exit_xform(r, h, pool);
   if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
@@ -590,7 +629,7 @@ __device__ __forceinline__ void store6(double* base, int64_t cs, const R6& r) {
   base[5 * cs] = r.ct;
 }
 
-__global__ void __launch_bounds__(256) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
+__global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
   cInstance* insts = (cInstance*)insts_g;
   // Instance of this workgroup: the last whose first block is <= blockIdx.x.
   int lo = 0, hi = ninst - 1;

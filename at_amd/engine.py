@@ -35,8 +35,17 @@ EXPORTS = (
     "atx_last_error", "atx_lattice_create", "atx_lattice_destroy", "atx_lattice_add_element",
     "atx_element_set_double", "atx_element_set_int", "atx_element_set_array",
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
-    "atx_lattice_turn_counter", "atx_device_count", "atx_device_info", "atx_set_device",
+    "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
+    "atx_set_device", "atx_lattice_set_integrator",
 )
+
+
+class BatchItem(C.Structure):
+    """struct atx_batch_item of include/atx.h."""
+
+    _fields_ = [("lattice", C.c_void_p), ("r", C.c_void_p), ("num_particles", C.c_uint32),
+                ("pad", C.c_uint32), ("lost_flag", C.c_void_p), ("lost_turn", C.c_void_p),
+                ("lost_elem", C.c_void_p), ("lost_coord", C.c_void_p)]
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -72,7 +81,11 @@ def load_library(path: str | None = None) -> C.CDLL:
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         lib.atx_elempass.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_double,
                                      C.c_double, C.c_void_p, C.c_uint32]
+        lib.atx_track_batch.argtypes = [C.POINTER(BatchItem), C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                        C.c_void_p, C.c_uint32]
         lib.atx_lattice_turn_counter.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.atx_lattice_set_integrator.argtypes = [C.c_void_p, C.c_int32]
         lib.atx_device_count.argtypes = [C.POINTER(C.c_int)]
         lib.atx_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.POINTER(C.c_int), C.c_char_p, C.c_int]
@@ -130,6 +143,7 @@ class DeviceLattice:
         _check(self.lib.atx_lattice_create(C.byref(h)), self.lib)
         self.h = h
         self.nelem = 0
+        self.integrator = 4
         try:
             for i, e in enumerate(elements):
                 pm = getattr(e, "PassMethod", None)
@@ -156,6 +170,11 @@ class DeviceLattice:
             self.close()
         except Exception:
             pass
+
+    def set_integrator(self, integrator: int) -> None:
+        """Integrator type 1-6 of atgpu's SymplecticIntegrator (4 = CPU)."""
+        _check(self.lib.atx_lattice_set_integrator(self.h, int(integrator)), self.lib)
+        self.integrator = int(integrator)
 
     def check(self, energy: float, rest_energy: float) -> None:
         _check(self.lib.atx_lattice_check(self.h, float(energy), float(rest_energy)), self.lib)
@@ -224,6 +243,29 @@ class DeviceLattice:
                                    ATX_STRICT if strict else 0)
         _check(rc, self.lib)
         return r_in
+
+
+def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
+                rest_energy: float = 0.0, losses: bool = True, block_size: int = 0,
+                stream: int | None = None, strict: bool = False, soa: bool = False) -> None:
+    """One launch over many (DeviceLattice, device particle buffer) pairs.
+
+    items: sequence of dicts with keys ``lattice`` (DeviceLattice), ``r``
+    (device pointer), ``n`` (particles) and, with losses, ``lost`` = (flag,
+    turn, elem, coord) device pointers."""
+    lib = load_library()
+    arr = (BatchItem * max(len(items), 1))()
+    for i, it in enumerate(items):
+        b = arr[i]
+        b.lattice = it["lattice"].h
+        b.r = it["r"]
+        b.num_particles = int(it["n"])
+        if losses:
+            b.lost_flag, b.lost_turn, b.lost_elem, b.lost_coord = it["lost"]
+    flags = (ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
+    _check(lib.atx_track_batch(arr, len(items), int(nturns), int(turn), float(energy),
+                               float(rest_energy), int(bool(losses)), int(block_size), stream,
+                               flags), lib)
 
 
 def device_count() -> int:

@@ -117,9 +117,6 @@ def gpupass(lattice, r_in, nturns: int, refpts=None, turn: int | None = None,
         raise TypeError("r_in must be a numpy array")
     if not isinstance(nturns, (int, np.integer)):
         raise TypeError("nturns must be an integer")
-    if integrator != 4:
-        raise RuntimeError("Only the 4th-order Forest-Ruth integrator (integrator=4) matches "
-                           "the reference CPU integrators")
     if gpu_pool is not None:
         if len(gpu_pool) == 0:
             raise RuntimeError("at_gpupass() gpu_pool is empty")
@@ -137,6 +134,10 @@ def gpupass(lattice, r_in, nturns: int, refpts=None, turn: int | None = None,
         _Cache.lattice_id = None
         _Cache.device_lattice = engine.DeviceLattice(lattice)
     dl = _Cache.device_lattice
+    if getattr(dl, "integrator", 4) != integrator:
+        if verbose and reuse:
+            print("Warning, lattice is recreated when integrator type is changed")
+        dl.set_integrator(integrator)
     if verbose:
         print(f"Lattice created [{dl.nelem} elements]; tracking {r_in.size // 6} particles "
               f"for {nturns} turns")

@@ -101,6 +101,90 @@ def large_ring(params: dict | None = None, radiation: bool = True, cavity: bool 
     return Lattice(elems, energy=p["energy"], particle=Particle("electron"), name="large_ring")
 
 
+def _truncnorm(rng, sigma: float, cut: float, size):
+    z = rng.standard_normal(size)
+    bad = np.abs(z) > cut
+    while bad.any():
+        z[bad] = rng.standard_normal(int(bad.sum()))
+        bad = np.abs(z) > cut
+    return sigma * z
+
+
+def _roll(theta: float) -> np.ndarray:
+    c, s = math.cos(theta), math.sin(theta)
+    m = np.eye(6)
+    m[0, 0] = m[1, 1] = m[2, 2] = m[3, 3] = c
+    m[0, 2] = m[1, 3] = s
+    m[2, 0] = m[3, 1] = -s
+    return m
+
+
+def error_realisation(ring: Lattice, seed: int, shift: float = 30e-6, roll: float = 100e-6,
+                      gradient: float = 1e-3, cut: float = 2.5) -> Lattice:
+    """Copy of ``ring`` with seeded errors on every quadrupole and sextupole:
+    transverse shifts (T1 = -d, T2 = +d), rolls about s (R1 = R(theta),
+    R2 = R(-theta)) and relative strength errors on PolynomB[1]
+    (quadrupoles) / PolynomB[2] (sextupoles), each Gaussian truncated at
+    ``cut`` sigma."""
+    rng = np.random.default_rng(seed)
+    elems = []
+    for e in ring:
+        if isinstance(e, (Quadrupole, Sextupole)):
+            e = e.deepcopy()
+            dx, dy = _truncnorm(rng, shift, cut, 2)
+            th = float(_truncnorm(rng, roll, cut, 1)[0])
+            dk = float(_truncnorm(rng, gradient, cut, 1)[0])
+            e.T1 = np.array([-dx, 0.0, -dy, 0.0, 0.0, 0.0])
+            e.T2 = np.array([dx, 0.0, dy, 0.0, 0.0, 0.0])
+            e.R1 = _roll(th)
+            e.R2 = _roll(-th)
+            pb = e.PolynomB.copy()
+            pb[2 if isinstance(e, Sextupole) else 1] *= 1.0 + dk
+            e.PolynomB = pb
+        elems.append(e)
+    lat = ring.copy()
+    list.__init__(lat, elems)
+    return lat
+
+
+def da_polar_grid(n: int, xmax: float = 15e-3, ymax: float = 6e-3,
+                  deltas=(-0.03, 0.0, 0.03)) -> np.ndarray:
+    """(6, n) F-ordered dynamic-aperture grid: for each momentum deviation a
+    radial grid r in [1e-6, 1] x theta in [0, pi], x = xmax r cos(theta),
+    y = ymax r sin(theta) (the RADIAL mode of pyat acceptance/boundary.py:
+    126-131), n points in total."""
+    per = [n // len(deltas) + (1 if i < n % len(deltas) else 0) for i in range(len(deltas))]
+    cols = []
+    for d, m in zip(deltas, per):
+        nth = max(2, int(round(math.sqrt(m))))
+        nr = max(1, int(math.ceil(m / nth)))
+        r = np.linspace(0.0, 1.0, nr)
+        r[r == 0] = 1e-6
+        th = np.linspace(0.0, math.pi, nth)
+        g1, g2 = np.meshgrid(r, th)
+        x = (xmax * np.cos(g2) * g1).ravel()[:m]
+        y = (ymax * np.sin(g2) * g1).ravel()[:m]
+        p = np.zeros((6, m))
+        p[0] = x
+        p[2] = y
+        p[4] = d
+        cols.append(p)
+    return np.asfortranarray(np.concatenate(cols, axis=1))
+
+
+def batched_da_workload(ninst: int = 64, seed: int = 7, nmin: int = 1000, nmax: int = 40000,
+                        ncells: int | None = None):
+    """Error-seeded copies of the large ring with DA grids of U[nmin, nmax]
+    particles each: a list of (lattice, particles)."""
+    rng = np.random.default_rng(seed)
+    base = large_ring(ncells=ncells)
+    out = []
+    for i in range(ninst):
+        n = int(rng.integers(nmin, nmax + 1))
+        out.append((error_realisation(base, seed * 1000 + i), da_polar_grid(n)))
+    return out
+
+
 def fodo_ring(ncells: int = 32, energy: float = 3.0e9) -> Lattice:
     """32 x [QF, D, B, D, QD, D, B, D, SF, SD], 6.0 m per cell."""
     angle = 2.0 * math.pi / (2 * ncells)

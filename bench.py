@@ -67,6 +67,9 @@ def main() -> int:
     ap.add_argument("--cpu-particles", type=int, default=32768)
     ap.add_argument("--cpu-turns", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (3 times): the "
+                         "calibration kernel for the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -76,6 +79,15 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * 131072, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8, "dispatches": 3}))
+        return 0
 
     from at_amd import engine
     from at_amd.flops import lattice_flops, tracked_flops_and_passes

@@ -62,6 +62,14 @@ int atx_element_set_double(atx_lattice *lat, const char *name, double value);
 int atx_element_set_int(atx_lattice *lat, const char *name, int64_t value);
 int atx_element_set_array(atx_lattice *lat, const char *name, const double *data, int64_t n);
 int atx_lattice_num_elements(const atx_lattice *lat);
+/* Symplectic integrator of the thick multipole slices, the `integrator`
+ * keyword of gpupass (atgpu/PyInterface.cpp:44-50,259-266) with the
+ * coefficient sets of atgpu/SymplecticIntegrator.cpp:32-86: 1 Euler,
+ * 2 McLachlan-Atela 2nd order, 3 Ruth 3rd order, 4 Forest-Ruth 4th order
+ * (default; the only one the CPU integrators use), 5 McLachlan-Atela 4th
+ * order, 6 Yoshida 6th order. Other values: ATX_E_RUNTIME. */
+int atx_lattice_set_integrator(atx_lattice *lat, int32_t type);
+
 /* Validate every element against its pass method (required attributes,
  * array sizes, energy where radiation or RF needs it) without tracking. */
 int atx_lattice_check(atx_lattice *lat, double energy, double rest_energy);
@@ -96,6 +104,30 @@ int atx_track(atx_lattice *lat, double *r_in, uint32_t num_particles, int32_t nt
  * (pyat/at.c:683-753): turn 0, ring length 0, no loss check. */
 int atx_elempass(atx_lattice *lat, int32_t index, double *r_in, uint32_t num_particles,
                  double energy, double rest_energy, void *stream, uint32_t flags);
+
+/* --- Batched instances ---------------------------------------------------
+ * Many lattice instances (e.g. error realisations), each with its own
+ * device-resident particles, tracked for the same number of turns in ONE
+ * launch: workgroups are assigned to instances in proportion to their
+ * particle counts. No counterpart in the reference, which tracks one
+ * lattice per call (at.c:340, PyInterface.cpp:132); each instance has the
+ * semantics of atx_track with refpts = [] and start/end = whole ring.
+ * All pointers are device pointers; the call is asynchronous on stream.
+ * block_size: workgroup size (64..512, multiple of 64; 0 = 256). */
+typedef struct atx_batch_item {
+    atx_lattice *lattice;
+    double *r;              /* (6,N) F-order, or SoA with ATX_SOA */
+    uint32_t num_particles;
+    uint32_t pad;
+    uint8_t *lost_flag;     /* N, or NULL when losses == 0 */
+    uint32_t *lost_turn;    /* N */
+    uint32_t *lost_elem;    /* N */
+    double *lost_coord;     /* (6,N) */
+} atx_batch_item;
+
+int atx_track_batch(const atx_batch_item *items, int32_t num_items, int32_t nturns, int32_t turn,
+                    double energy, double rest_energy, int32_t losses, int32_t block_size,
+                    void *stream, uint32_t flags);
 
 /* Turn counter after the last call (at.c:656 last_turn). */
 int atx_lattice_turn_counter(const atx_lattice *lat, int64_t *turn);

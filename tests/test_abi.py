@@ -113,5 +113,27 @@ def test_refpts_normalisation():
     assert get_uint32_index(lat, 2).dtype == np.uint32
 
 
+def test_integrator_types():
+    # atgpu/SymplecticIntegrator.cpp:32-86: types 1..6, others rejected
+    lat = engine.DeviceLattice([Quadrupole("q", 0.5, 1.0)])
+    for t in range(1, 7):
+        lat.set_integrator(t)
+        lat.check(0.0, 0.0)
+    with pytest.raises(RuntimeError, match="not supported"):
+        lat.set_integrator(7)
+
+
+def test_batch_argument_errors():
+    import ctypes as C
+
+    lib = engine.load_library()
+    item = engine.BatchItem()
+    item.lattice = None
+    item.r = None
+    item.num_particles = 10
+    rc = lib.atx_track_batch(C.byref(item), 1, 5, 0, 3e9, 0.0, 0, 0, None, 0)
+    assert rc == engine.ATX_E_VALUE
+
+
 def test_device_count_without_gpu_is_zero_or_more():
     assert engine.device_count() >= 0
