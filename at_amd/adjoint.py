@@ -1,0 +1,139 @@
+"""Gradients of a tracking loss with respect to magnet strengths.
+
+``TrackingLoss`` is a torch.autograd.Function whose forward tracks the
+particles through ``nturns`` turns on the engine and returns
+sum over surviving particles of x^2 + px^2 + y^2 + py^2 at the end; its
+backward is the hand-written adjoint kernel (atx_adjoint):
+
+* forward keeps one checkpoint of the particle state every ``checkpoint``
+  turns (8 by default);
+* backward walks the blocks from the last: it recomputes the block forward
+  from its checkpoint with the tracking kernel, recording the state at the
+  start of every turn (refpt 0), then runs the adjoint kernel over the
+  block, which reconstructs every element's input by exact inversion and
+  accumulates dL/dPolynomB[k] of the selected elements.
+
+Memory: checkpoints (nturns/checkpoint + 1) x N x 48 B plus one block of
+turn-start states (checkpoint + 1) x N x 48 B.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import engine
+
+
+class Problem:
+    """A lattice, its differentiation parameters and the tracking setup.
+
+    params: sequence of (element index, PolynomB index)."""
+
+    def __init__(self, lattice, params, *, energy=None, rest_energy=None, checkpoint: int = 8):
+        self.lattice = lattice
+        self.params = [(int(e), int(k)) for e, k in params]
+        self.energy = float(lattice.energy if energy is None else energy)
+        part = getattr(lattice, "particle", None)
+        self.rest = float(getattr(part, "rest_energy", 0.0) if rest_energy is None else rest_energy)
+        self.checkpoint = int(checkpoint)
+        self.dl = engine.DeviceLattice(lattice)
+        n = len(lattice)
+        self.slots = np.full(n, -1, np.int32)
+        self.coeffs = np.zeros(n, np.int32)
+        for i, (e, k) in enumerate(self.params):
+            if self.slots[e] >= 0:
+                raise ValueError(f"element {e} appears twice in params")
+            self.slots[e] = i
+            self.coeffs[e] = k
+        self._pb = {e: np.array(lattice[e].PolynomB, dtype=np.float64) for e, _ in self.params}
+
+    def initial_strengths(self) -> np.ndarray:
+        return np.array([self._pb[e][k] for e, k in self.params])
+
+    def set_strengths(self, values) -> None:
+        vals = np.asarray(values, dtype=np.float64)
+        touched = {}
+        for (e, k), v in zip(self.params, vals):
+            pb = touched.setdefault(e, self._pb[e].copy())
+            pb[k] = v
+        for e, pb in touched.items():
+            self.dl.set_array_at(e, "PolynomB", pb)
+
+
+def _loss_and_seed(r_end):
+    """Loss of the end state and its adjoint (zero for lost particles)."""
+    import torch
+
+    alive = torch.isfinite(r_end).all(dim=1)
+    q = r_end[:, :4]
+    loss = torch.where(alive[:, None], q * q, torch.zeros_like(q)).sum()
+    lam = torch.zeros_like(r_end)
+    lam[:, :4] = torch.where(alive[:, None], 2.0 * q, torch.zeros_like(q))
+    return loss, lam
+
+
+class TrackingLoss:
+    """Differentiable L(strengths) = sum_alive (x^2 + px^2 + y^2 + py^2)."""
+
+    @staticmethod
+    def apply(problem: Problem, strengths, particles, nturns: int):
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, theta):
+                problem.set_strengths(theta.detach().cpu().numpy())
+                r = particles.detach().clone()
+                n = r.shape[0]
+                cp = problem.checkpoint
+                ckpts = []
+                stream = torch.cuda.current_stream().cuda_stream
+                t = 0
+                while t < nturns:
+                    nt = min(cp, nturns - t)
+                    ckpts.append((t, nt, r.clone()))
+                    problem.dl.track_device(r.data_ptr(), n, nt, turn=t, energy=problem.energy,
+                                            rest_energy=problem.rest, stream=stream)
+                    t += nt
+                loss, lam = _loss_and_seed(r)
+                ctx.ckpts = ckpts
+                ctx.lam = lam
+                ctx.theta_device = theta.device
+                ctx.final = r
+                return loss
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                n = particles.shape[0]
+                npar = len(problem.params)
+                grad = torch.zeros(npar, dtype=torch.float64, device=particles.device)
+                lam = (ctx.lam * grad_out).contiguous()
+                stream = torch.cuda.current_stream().cuda_stream
+                for t0, nt, ck in reversed(ctx.ckpts):
+                    r = ck.clone()
+                    states = torch.empty((nt + 1, n, 6), dtype=torch.float64,
+                                         device=particles.device)
+                    problem.dl.track_device(r.data_ptr(), n, nt, turn=t0,
+                                            energy=problem.energy, rest_energy=problem.rest,
+                                            refpts=np.array([0], np.uint32),
+                                            rout_ptr=states.data_ptr(), stream=stream)
+                    states[nt].copy_(r)
+                    problem.dl.adjoint(states.data_ptr(), nt, t0, n, problem.energy,
+                                       problem.rest, lam.data_ptr(), problem.slots,
+                                       problem.coeffs, grad.data_ptr(), npar, stream=stream)
+                return grad.to(device=ctx.theta_device, dtype=torch.float64)
+
+        return _F.apply(strengths)
+
+
+def loss_and_grad(problem: Problem, particles_np: np.ndarray, nturns: int, strengths=None):
+    """Convenience: (loss, dL/dstrengths) as numpy for a (6, N) beam."""
+    import torch
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    parts = torch.from_numpy(np.ascontiguousarray(np.asarray(particles_np).T)).to(dev)
+    theta0 = problem.initial_strengths() if strengths is None else np.asarray(strengths)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    loss = TrackingLoss.apply(problem, theta, parts, nturns)
+    loss.backward()
+    return float(loss.detach().cpu()), theta.grad.detach().cpu().numpy()

@@ -49,13 +49,19 @@ def _check_particles(p):
 
 def track_batch(lattices, particles, nturns: int, *, turn: int = 0, energy: float,
                 rest_energy: float, losses: bool = True, loss_maps=None, block_size: int = 0,
-                strict: bool = False, stream=None):
+                strict: bool = False, stream=None, item_turns=None,
+                continue_losses: bool = False):
     """Track every (lattice, particles) pair for ``nturns`` in one launch,
-    in place. Returns the list of LossMap when ``losses``."""
+    in place. ``item_turns``, if given, holds one (nturns, turn) pair per
+    instance, which then replaces the common ``nturns``/``turn``. With
+    ``continue_losses`` the given loss maps carry over from an earlier call on
+    the same particles. Returns the list of LossMap when ``losses``."""
     import torch
 
     if len(lattices) != len(particles):
         raise ValueError("one particle tensor per lattice")
+    if item_turns is not None and len(item_turns) != len(lattices):
+        raise ValueError("one (nturns, turn) pair per lattice")
     for p in particles:
         _check_particles(p)
     if losses and loss_maps is None:
@@ -63,11 +69,15 @@ def track_batch(lattices, particles, nturns: int, *, turn: int = 0, energy: floa
     items = []
     for i, (lat, p) in enumerate(zip(lattices, particles)):
         it = {"lattice": lat, "r": p.data_ptr(), "n": p.shape[0]}
+        if item_turns is not None:
+            it["nturns"], it["turn"] = (int(v) for v in item_turns[i])
+            if it["nturns"] < 0:
+                raise ValueError("item turn counts must be >= 0")
         if losses:
             it["lost"] = loss_maps[i].ptrs()
         items.append(it)
     s = stream if stream is not None else torch.cuda.current_stream()
     engine.track_batch(items, nturns, turn=turn, energy=energy, rest_energy=rest_energy,
                        losses=losses, block_size=block_size, stream=s.cuda_stream,
-                       strict=strict)
+                       strict=strict, continue_losses=continue_losses)
     return loss_maps if losses else None

@@ -51,6 +51,10 @@ def build(verbose_isa: bool = False) -> str:
             cmd += ["-Rpass-analysis=kernel-resource-usage"]
         _run(cmd)
         objs.append(obj)
+    adj_obj = os.path.join(BUILD, "atx_adjoint.o")
+    _run([cc, "-c", *COMMON, "-ffp-contract=fast", os.path.join(CSRC, "atx_adjoint.hip"), "-o",
+          adj_obj] + (["-Rpass-analysis=kernel-resource-usage"] if verbose_isa else []))
+    objs.append(adj_obj)
     host_obj = os.path.join(BUILD, "atx_host.o")
     _run([cc, "-c", *COMMON, "-ffp-contract=off", os.path.join(CSRC, "atx_host.cpp"), "-o", host_obj])
     objs.append(host_obj)

@@ -122,9 +122,33 @@ struct Instance {
   int32_t n_first;          // copies made on the first turn
   int32_t n_std;            // copies made on each later turn
   int32_t block_begin;      // first workgroup of this instance
-  int32_t losses;           // 1: record the loss map (at.c checkiflost); 0: setlost only
+  int32_t losses;           // 1: record the loss map (at.c checkiflost); 0: setlost only;
+                            // 2: as 1, continuing the map of an earlier call
   int32_t elempass;         // 1: single element pass, no loss check (at.c at_elempass)
   int32_t pad;
+};
+
+// Per-element data of the adjoint kernel: gradient slot (-1: none) and
+// PolynomB index of the differentiated coefficient; pool offsets of the
+// inverse misalignment matrices.
+struct AdjElem {
+  int32_t slot;
+  int32_t coeff;
+  int32_t r1inv;
+  int32_t r2inv;
+};
+
+struct AdjArgs {
+  const DevElem* elems;
+  const double* pool;
+  const AdjElem* adj;
+  const double* states;  // (nturns+1) blocks of (6, N): turn-start states, then the end state
+  double* lam;           // (6, N) adjoint of the end state in, of the start state out
+  double* grad;          // accumulated dL/dPolynomB[coeff] per slot
+  int32_t nelem;
+  int32_t npart;
+  int32_t nturns;
+  int32_t turn0;         // turn counter of the first turn of the block
 };
 
 }  // namespace atx

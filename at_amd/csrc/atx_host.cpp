@@ -18,6 +18,7 @@
 
 extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, hipStream_t);
 
 namespace atx {
 
@@ -84,7 +85,46 @@ struct Compiled {
   std::vector<DevElem> elems;
   std::vector<double> pool;
   double T0 = 0.0;
+  std::vector<int32_t> r1inv, r2inv;  // per element, pool offsets (-1: none)
+  std::vector<std::string> adjerr;    // per element, why the adjoint cannot run it ("" = it can)
 };
+
+// Gauss-Jordan inverse of a 6x6 column-major matrix with partial pivoting.
+bool invert6(const double* m, double* inv) {
+  double a[6][12];
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) {
+      a[i][j] = m[i + 6 * j];
+      a[i][j + 6] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 6; c++) {
+    int p = c;
+    for (int i = c + 1; i < 6; i++)
+      if (std::fabs(a[i][c]) > std::fabs(a[p][c])) p = i;
+    if (a[p][c] == 0.0) return false;
+    if (p != c)
+      for (int j = 0; j < 12; j++) std::swap(a[p][j], a[c][j]);
+    const double d = a[c][c];
+    for (int j = 0; j < 12; j++) a[c][j] /= d;
+    for (int i = 0; i < 6; i++) {
+      if (i == c) continue;
+      const double f = a[i][c];
+      for (int j = 0; j < 12; j++) a[i][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) inv[i + 6 * j] = a[i][j + 6];
+  return true;
+}
+
+// row and column 4 (delta) of the matrix are those of the identity
+bool delta_preserving(const double* m) {
+  for (int k = 0; k < 6; k++) {
+    const double e = (k == 4) ? 1.0 : 0.0;
+    if (m[4 + 6 * k] != e || m[k + 6 * 4] != e) return false;
+  }
+  return true;
+}
 
 // Drift (c) and kick (d) coefficients per slice of the integrator types of
 // atgpu/SymplecticIntegrator.cpp:32-86 (published coefficient sets: Euler;
@@ -133,6 +173,9 @@ struct atx_lattice {
   size_t d_elems_n = 0, d_pool_n = 0;
   int64_t turn_counter = 0;
   int integrator = 4;
+  // host copies of per-element compile results used by the adjoint
+  std::vector<int32_t> h_r1inv, h_r2inv, h_order, h_kind;
+  std::vector<std::string> h_adjerr;
   std::mutex mu;
 };
 
@@ -435,6 +478,35 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
   }
   if (rd.err) return rd.err;
   h.sc = push(pool, sc, S_NSCALAR);
+  // adjoint support: inverse misalignment matrices, and whether the reverse
+  // kernel can invert this element in closed form
+  int32_t r1i = -1, r2i = -1;
+  std::string aerr;
+  if (h.flags & F_R1) {
+    double inv[36];
+    if (!invert6(&pool[h.r1], inv)) aerr = "singular R1";
+    r1i = push(pool, inv, 36);
+    if (!delta_preserving(&pool[h.r1])) aerr = "R1 couples to delta";
+  }
+  if (h.flags & F_R2) {
+    double inv[36];
+    if (!invert6(&pool[h.r2], inv)) aerr = "singular R2";
+    r2i = push(pool, inv, 36);
+    if (!delta_preserving(&pool[h.r2])) aerr = "R2 couples to delta";
+  }
+  if ((h.flags & F_T1) && pool[h.t1 + 4] != 0.0) aerr = "T1 shifts delta";
+  if ((h.flags & F_T2) && pool[h.t2 + 4] != 0.0) aerr = "T2 shifts delta";
+  if (h.kind == K_STRRAD || h.kind == K_BNDRAD) aerr = "radiating pass";
+  if (h.flags & F_SCALE) aerr = "FieldScaling != 1";
+  if (h.fringe) aerr = "quadrupole fringe";
+  if (h.ops >= 0) aerr = "integrator other than 4";
+  for (int side = 0; side < 2; side++) {
+    const int m = (h.edge >> (8 * side)) & 0xff;
+    if ((h.kind == K_BND) && m != E_CONST && m != E_DIVP) aerr = "edge fringe with gap correction";
+  }
+  out.r1inv.push_back(r1i);
+  out.r2inv.push_back(r2i);
+  out.adjerr.push_back(aerr);
   out.elems.push_back(h);
   return ATX_OK;
 }
@@ -503,6 +575,15 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
   HIPCHK(hipMemcpy(lat->d_pool, c.pool.data(), c.pool.size() * sizeof(double), hipMemcpyHostToDevice));
   lat->d_elems_n = c.elems.size();
   lat->d_pool_n = c.pool.size();
+  lat->h_r1inv = c.r1inv;
+  lat->h_r2inv = c.r2inv;
+  lat->h_adjerr = c.adjerr;
+  lat->h_order.clear();
+  lat->h_kind.clear();
+  for (const auto& h : c.elems) {
+    lat->h_order.push_back(h.order);
+    lat->h_kind.push_back(h.kind);
+  }
   lat->dev_valid = true;
   lat->dev_energy = energy;
   lat->dev_rest = rest;
@@ -516,7 +597,9 @@ constexpr int kBlock = 256;
 // Launch one kernel over several instances: instances get consecutive
 // workgroup ranges; one stream-ordered scratch block carries the instance
 // table and every instance's refpt slot table.
-int launch_instances(std::vector<Instance>& insts, const std::vector<std::vector<int32_t>>& slots,
+// slots[i] is the refpt slot table of instance i; instances that share a
+// table (same pointer) share its device copy.
+int launch_instances(std::vector<Instance>& insts, const std::vector<const std::vector<int32_t>*>& slots,
                      int block, hipStream_t stream, bool strict) {
   if (block <= 0) block = kBlock;
   if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
@@ -528,17 +611,24 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<std::vector
   if (nblocks == 0) return ATX_OK;
   size_t bytes = insts.size() * sizeof(Instance);
   std::vector<size_t> soff(insts.size());
+  std::map<const std::vector<int32_t>*, size_t> placed;
+  std::vector<std::pair<size_t, const std::vector<int32_t>*>> copies;
   for (size_t i = 0; i < insts.size(); i++) {
+    auto f = placed.find(slots[i]);
+    if (f != placed.end()) {
+      soff[i] = f->second;
+      continue;
+    }
     soff[i] = bytes;
-    bytes += slots[i].size() * sizeof(int32_t);
+    placed[slots[i]] = bytes;
+    copies.emplace_back(bytes, slots[i]);
+    bytes += slots[i]->size() * sizeof(int32_t);
   }
   char* d_scratch = nullptr;
   HIPCHK(hipMallocAsync((void**)&d_scratch, bytes, stream));
   std::vector<char> host(bytes);
-  for (size_t i = 0; i < insts.size(); i++) {
-    insts[i].slots = (const int32_t*)(d_scratch + soff[i]);
-    std::memcpy(host.data() + soff[i], slots[i].data(), slots[i].size() * sizeof(int32_t));
-  }
+  for (size_t i = 0; i < insts.size(); i++) insts[i].slots = (const int32_t*)(d_scratch + soff[i]);
+  for (auto& c : copies) std::memcpy(host.data() + c.first, c.second->data(), c.second->size() * sizeof(int32_t));
   std::memcpy(host.data(), insts.data(), insts.size() * sizeof(Instance));
   hipError_t ce = hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream);
   hipError_t le = hipSuccess;
@@ -591,7 +681,7 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   inst.block_begin = 0;
   inst.losses = losses ? 1 : 0;
   inst.elempass = elempass ? 1 : 0;
-  return launch_instances(insts, slots, kBlock, stream, strict);
+  return launch_instances(insts, {&slots[0]}, kBlock, stream, strict);
 }
 
 int check_partial(int nelem, int nturns, int& start, int& end) {
@@ -815,8 +905,12 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
   hipStream_t stream = (hipStream_t)stream_;
   const bool strict = (flags & ATX_STRICT) != 0;
   const bool soa = (flags & ATX_SOA) != 0;
+  for (int i = 0; i < nitems; i++)
+    if (items[i].nturns >= 0 && items[i].turn < 0)
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": turn must be >= 0");
   std::vector<Instance> insts;
-  std::vector<std::vector<int32_t>> slots;
+  std::map<int, std::vector<int32_t>> empty_slots;  // refpts = [] table per ring length
+  std::vector<const std::vector<int32_t>*> slots;
   std::vector<std::unique_lock<std::mutex>> locks;
   for (int i = 0; i < nitems; i++) {
     const atx_batch_item& it = items[i];
@@ -844,16 +938,104 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     in.cstride = soa ? (int64_t)it.num_particles : 1;
     in.nelem = nelem;
     in.npart = (int32_t)it.num_particles;
-    in.nturns = nturns;
-    in.turn0 = turn;
+    in.nturns = it.nturns >= 0 ? it.nturns : nturns;
+    in.turn0 = it.nturns >= 0 ? it.turn : turn;
     in.start_elem = 0;
     in.end_elem = nelem;
-    in.losses = losses ? 1 : 0;
+    in.losses = losses ? ((flags & ATX_CONTINUE_LOSSES) ? 2 : 1) : 0;
+    if (in.nturns == 0) continue;
     insts.push_back(in);
-    slots.emplace_back(2 * (size_t)(nelem + 1), -1);
+    auto ins = empty_slots.emplace(nelem, std::vector<int32_t>(2 * (size_t)(nelem + 1), -1));
+    slots.push_back(&ins.first->second);
   }
-  if (nturns == 0 || insts.empty()) return ATX_OK;
-  return launch_instances(insts, slots, block_size, stream, strict);
+  if (insts.empty()) return ATX_OK;
+  // Longest work per workgroup first: the hardware dispatches workgroups in
+  // index order, so long instances start early and short ones fill the tail.
+  std::vector<size_t> order(insts.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return (int64_t)insts[a].nturns * insts[a].nelem > (int64_t)insts[b].nturns * insts[b].nelem;
+  });
+  std::vector<Instance> sorted_insts;
+  std::vector<const std::vector<int32_t>*> sorted_slots;
+  sorted_insts.reserve(insts.size());
+  sorted_slots.reserve(insts.size());
+  for (size_t i : order) {
+    sorted_insts.push_back(insts[i]);
+    sorted_slots.push_back(slots[i]);
+  }
+  return launch_instances(sorted_insts, sorted_slots, block_size, stream, strict);
+}
+
+int atx_element_set_array_at(atx_lattice* lat, int32_t index, const char* name, const double* data,
+                             int64_t n) {
+  if (!lat || !name) return fail(ATX_E_VALUE, "NULL argument");
+  if (n < 0 || (n > 0 && !data)) return fail(ATX_E_VALUE, "bad array");
+  std::lock_guard<std::mutex> g(lat->mu);
+  if (index < 0 || index >= (int32_t)lat->elems.size()) return fail(ATX_E_VALUE, "element index out of range");
+  Attr a;
+  a.v.assign(data, data + n);
+  lat->elems[index].attrs[name] = std::move(a);
+  lat->dev_valid = false;
+  return ATX_OK;
+}
+
+int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t turn, uint32_t np,
+                double energy, double rest_energy, double* lam, const int32_t* grad_slot,
+                const int32_t* grad_coeff, double* grad, int32_t num_params, void* stream_) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  if (np > 0 && (!states || !lam)) return fail(ATX_E_VALUE, "states/lam are NULL");
+  std::lock_guard<std::mutex> g(lat->mu);
+  const int nelem = (int)lat->elems.size();
+  int rc = ensure_device_lattice(lat, energy, rest_energy, false);
+  if (rc) return rc;
+  std::vector<AdjElem> ae(nelem > 0 ? nelem : 1);
+  for (int e = 0; e < nelem; e++) {
+    if (!lat->h_adjerr[e].empty())
+      return fail(ATX_E_RUNTIME, "adjoint not available for element #" + std::to_string(e) + " " +
+                                     lat->elems[e].fam + " (" + lat->elems[e].pass + "): " + lat->h_adjerr[e]);
+    const int slot = grad_slot ? grad_slot[e] : -1;
+    const int coeff = grad_coeff ? grad_coeff[e] : 0;
+    if (slot >= 0) {
+      const int k = lat->h_kind[e];
+      if (k != K_STR && k != K_BND && k != K_THIN)
+        return fail(ATX_E_VALUE, "element #" + std::to_string(e) + " has no PolynomB to differentiate");
+      if (slot >= num_params || !grad) return fail(ATX_E_VALUE, "gradient slot out of range");
+      if (coeff < 0 || coeff > lat->h_order[e])
+        return fail(ATX_E_VALUE, "element #" + std::to_string(e) + ": PolynomB index above MaxOrder");
+    }
+    ae[e].slot = slot;
+    ae[e].coeff = coeff;
+    ae[e].r1inv = lat->h_r1inv[e];
+    ae[e].r2inv = lat->h_r2inv[e];
+  }
+  if (np == 0 || nturns == 0) return ATX_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  AdjElem* d_ae = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d_ae, ae.size() * sizeof(AdjElem), stream));
+  hipError_t ce = hipMemcpyAsync(d_ae, ae.data(), ae.size() * sizeof(AdjElem), hipMemcpyHostToDevice, stream);
+  hipError_t le = hipSuccess;
+  if (ce == hipSuccess) {
+    AdjArgs a;
+    a.elems = lat->d_elems;
+    a.pool = lat->d_pool;
+    a.adj = d_ae;
+    a.states = states;
+    a.lam = lam;
+    a.grad = grad;
+    a.nelem = nelem;
+    a.npart = (int32_t)np;
+    a.nturns = nturns;
+    a.turn0 = turn;
+    le = atx_launch_adjoint(&a, stream);
+  }
+  hipError_t fe = hipFreeAsync(d_ae, stream);
+  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("adjoint launch: ") + hipGetErrorString(le));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  return ATX_OK;
 }
 
 int atx_elempass(atx_lattice* lat, int32_t index, double* r_in, uint32_t np, double energy,

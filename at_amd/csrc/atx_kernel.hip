@@ -197,25 +197,59 @@ __device__ __forceinline__ void bnd_kick(R6& r, cdouble* a, cdouble* b, int n, d
 #endif
 }
 
-// Radiating drift (diff_drift.c:30-43). The fast build returns pn for the
-// following kick, which needs 1/(1+delta) at the same delta.
-__device__ __forceinline__ double rad_drift(R6& r, double len) {
+// Radiating drift (diff_drift.c:30-43).
+__device__ __forceinline__ void rad_drift(R6& r, double len) {
   const double pn = 1.0 / (1.0 + r.dp);
   const double nl = len * pn;
   r.x += nl * r.px;
   r.y += nl * r.py;
   r.ct += nl * pn * (r.px * r.px + r.py * r.py) / 2.0;
-  return pn;
 }
 
+#if !ATX_STRICT
+// Fast build: inside a radiating magnet delta changes only in the kicks, by
+// the radiated fraction (1e-8 or less per kick in storage rings), so
+// pn = 1/(1+delta) is carried from step to step and refreshed after each
+// kick by two Newton steps from the previous value,
+// pn' = pn (1 + e), e = 1 - (1+delta') pn: the error after the two steps is
+// e^4 plus rounding, within an ulp for |e| < 1e-4.
+__device__ __forceinline__ double newton_pn(double pn, double opd) {
+  double e = __builtin_fma(-opd, pn, 1.0);
+  pn = __builtin_fma(pn, e, pn);
+  e = __builtin_fma(-opd, pn, 1.0);
+  return __builtin_fma(pn, e, pn);
+}
+
+// 1/d from the hardware reciprocal estimate and two Newton steps (the
+// estimate's relative error, below 2^-20, is squared twice).
+__device__ __forceinline__ double fast_rcp(double d) {
+  double q = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, q, 1.0);
+  q = __builtin_fma(q, e, q);
+  e = __builtin_fma(-d, q, 1.0);
+  return __builtin_fma(q, e, q);
+}
+
+// Radiating drift with the carried pn.
+__device__ __forceinline__ void rad_drift_pn(R6& r, double len, double pn) {
+  const double nl = len * pn;
+  r.x = __builtin_fma(nl, r.px, r.x);
+  r.y = __builtin_fma(nl, r.py, r.y);
+  const double p2 = __builtin_fma(r.px, r.px, r.py * r.py);
+  r.ct = __builtin_fma(nl * pn * 0.5, p2, r.ct);
+}
+#endif
+
 // Radiating straight kick (diff_str_kick.c:19-83, bdiff == NULL).
+// The fast build takes pn = 1/(1+delta) at entry and returns it updated to
+// the radiated delta.
 template <int N>
 __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                             double radc, double pn_in) {
+                                             double radc, double& pn_io) {
   double re, im;
   field<N>(r, a, b, n, re, im);
 #if ATX_STRICT
-  (void)pn_in;
+  (void)pn_io;
   double pn = 1.0 / (1.0 + r.dp);
   const double xpr = r.px * pn;
   const double ypr = r.py * pn;
@@ -230,11 +264,12 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
   r.px -= kl * re;
   r.py += kl * im;
 #else
+  const double pn_in = pn_io;
   const double opd = 1.0 + r.dp;
   const double xpr = r.px * pn_in;
   const double ypr = r.py * pn_in;
   const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
-  const double vn2 = 1.0 / (1.0 + s2);
+  const double vn2 = fast_rcp(1.0 + s2);
   const double cr = __builtin_fma(im, ypr, -re * xpr);
   const double b2p = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im)) * vn2;
   const double factor = __builtin_fma(0.5, s2, 1.0) * (opd * opd) * kl;
@@ -242,6 +277,7 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
   const double opd2 = 1.0 + r.dp;
   r.px = __builtin_fma(xpr, opd2, -kl * re);
   r.py = __builtin_fma(ypr, opd2, kl * im);
+  pn_io = newton_pn(pn_in, opd2);
 #endif
 }
 
@@ -249,11 +285,11 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
 template <int N>
 __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
                                              double irho, double kli, double radc,
-                                             double pn_in) {
+                                             double& pn_io) {
   double re, im;
   field<N>(r, a, b, n, re, im);
 #if ATX_STRICT
-  (void)pn_in;
+  (void)pn_io;
   double pn = 1.0 / (1.0 + r.dp);
   const double x = r.x;
   const double xpr = r.px * pn;
@@ -276,6 +312,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   r.py += kl * im;
   r.ct += kli * r.x;
 #else
+  const double pn_in = pn_io;
   const double x = r.x;
   const double opd = 1.0 + r.dp;
   const double xpr = r.px * pn_in;
@@ -283,7 +320,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   const double by = re + irho;
   const double hx = __builtin_fma(x, irho, 1.0);
   const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
-  const double vn2 = 1.0 / __builtin_fma(hx, hx, s2);
+  const double vn2 = fast_rcp(__builtin_fma(hx, hx, s2));
   const double byh = by * hx;
   const double bxh = im * hx;
   const double cr = __builtin_fma(im, ypr, -by * xpr);
@@ -297,6 +334,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   r.px = __builtin_fma(xpr, opd2, -kl * v);
   r.py = __builtin_fma(ypr, opd2, kl * im);
   r.ct = __builtin_fma(kli, x, r.ct);
+  pn_io = newton_pn(pn_in, opd2);
 #endif
 }
 
@@ -413,7 +451,7 @@ __device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdoub
         if (RAD) rad_drift(r, len); else fastdrift(r, len * pn0, h2);
       } else {
         if (RAD) {
-          const double pn = 1.0 / (1.0 + r.dp);
+          double pn = 1.0 / (1.0 + r.dp);
           if (BEND) rad_bnd_kick<-1>(r, a, b, n, len, irho, len * irho, radc, pn);
           else rad_str_kick<-1>(r, a, b, n, len, radc, pn);
         } else {
@@ -487,18 +525,35 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
   } else {
     const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
     const double ki1 = k1 * irho, ki2 = k2 * irho;
+#if ATX_STRICT
+    double pn = 0.0;  // unused: the strict steps divide as the reference does
     for (int m = 0; m < h.nslice; m++) {
-      double pn = rad_drift(r, l1);
+      rad_drift(r, l1);
       if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
       else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
-      pn = rad_drift(r, l2);
+      rad_drift(r, l2);
       if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
       else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
-      pn = rad_drift(r, l2);
+      rad_drift(r, l2);
       if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
       else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
       rad_drift(r, l1);
     }
+#else
+    double pn = 1.0 / (1.0 + r.dp);  // one division per magnet
+    for (int m = 0; m < h.nslice; m++) {
+      rad_drift_pn(r, l1, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      rad_drift_pn(r, l2, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
+      rad_drift_pn(r, l2, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      rad_drift_pn(r, l1, pn);
+    }
+#endif
   }
   if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
   if (BEND) edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
@@ -661,14 +716,16 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   const bool check = !I.elempass;
   double* rout = I.rout;
   const int64_t np6 = 6 * (int64_t)npart;
-  if (losses) {
+  bool lost = false;
+  if (losses == 2) {
+    lost = I.lost_flag[p] != 0;  // recorded by an earlier call
+  } else if (losses) {
     I.lost_flag[p] = 0;
     I.lost_turn[p] = 0;
     I.lost_elem[p] = 0;
     R6 z = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     store6(I.lost_coord + 6 * (int64_t)p, 1, z);
   }
-  bool lost = false;
   const int nturns = I.nturns;
   for (int t = 0; t < nturns; t++) {
     const int nturn = I.turn0 + t;

@@ -29,6 +29,7 @@ ATX_DEVICE_POINTERS = 0x1
 ATX_STRICT = 0x2
 ATX_KEEP_COUNTER = 0x4
 ATX_SOA = 0x8
+ATX_CONTINUE_LOSSES = 0x10
 
 # Exported symbols, as declared in include/atx.h.
 EXPORTS = (
@@ -36,7 +37,7 @@ EXPORTS = (
     "atx_element_set_double", "atx_element_set_int", "atx_element_set_array",
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
-    "atx_set_device", "atx_lattice_set_integrator",
+    "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
 )
 
 
@@ -45,7 +46,8 @@ class BatchItem(C.Structure):
 
     _fields_ = [("lattice", C.c_void_p), ("r", C.c_void_p), ("num_particles", C.c_uint32),
                 ("pad", C.c_uint32), ("lost_flag", C.c_void_p), ("lost_turn", C.c_void_p),
-                ("lost_elem", C.c_void_p), ("lost_coord", C.c_void_p)]
+                ("lost_elem", C.c_void_p), ("lost_coord", C.c_void_p), ("nturns", C.c_int32),
+                ("turn", C.c_int32)]
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -53,6 +55,22 @@ _u8p = C.POINTER(C.c_uint8)
 
 _lib = None
 _lib_lock = threading.Lock()
+
+
+def _bind_torch_runtime() -> None:
+    """Keep one HIP runtime in the process.
+
+    PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7, which
+    torch requests as "libamdhip64.so"). If torch is loaded first,
+    libatx.so's request for libamdhip64.so.7 resolves to that same runtime,
+    so torch's device pointers and streams are valid in the engine. If the
+    engine were loaded first, torch would load a second runtime beside it,
+    and that one fails to initialise ("No HIP GPUs are available"). Torch is
+    optional: without it the engine uses the system runtime alone."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def load_library(path: str | None = None) -> C.CDLL:
@@ -65,6 +83,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         p = path or LIB_PATH
         if not os.path.exists(p):
             raise ImportError(f"at_amd engine library not built: {p} (run python -m at_amd.build)")
+        _bind_torch_runtime()
         lib = C.CDLL(p)
         lib.atx_last_error.restype = C.c_char_p
         lib.atx_lattice_create.argtypes = [C.POINTER(C.c_void_p)]
@@ -86,6 +105,12 @@ def load_library(path: str | None = None) -> C.CDLL:
                                         C.c_void_p, C.c_uint32]
         lib.atx_lattice_turn_counter.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.atx_lattice_set_integrator.argtypes = [C.c_void_p, C.c_int32]
+        lib.atx_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint32,
+                                    C.c_double, C.c_double, C.c_void_p,
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_void_p,
+                                    C.c_int32, C.c_void_p]
+        lib.atx_element_set_array_at.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, _dp,
+                                                 C.c_int64]
         lib.atx_device_count.argtypes = [C.POINTER(C.c_int)]
         lib.atx_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.POINTER(C.c_int), C.c_char_p, C.c_int]
@@ -176,6 +201,25 @@ class DeviceLattice:
         _check(self.lib.atx_lattice_set_integrator(self.h, int(integrator)), self.lib)
         self.integrator = int(integrator)
 
+    def set_array_at(self, index: int, name: str, values) -> None:
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel(order="F"))
+        _check(self.lib.atx_element_set_array_at(self.h, int(index), name.encode(),
+                                                 a.ctypes.data_as(_dp), a.size), self.lib)
+
+    def adjoint(self, states_ptr: int, nturns: int, turn: int, npart: int, energy: float,
+                rest_energy: float, lam_ptr: int, slots: np.ndarray, coeffs: np.ndarray,
+                grad_ptr: int, nparams: int, stream: int | None = None) -> None:
+        """Reverse-mode pass over one block of turns (atx_adjoint)."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        coeffs = np.ascontiguousarray(coeffs, np.int32)
+        if slots.size != self.nelem or coeffs.size != self.nelem:
+            raise ValueError("grad_slot/grad_coeff need one entry per element")
+        ip = C.POINTER(C.c_int32)
+        _check(self.lib.atx_adjoint(self.h, states_ptr, int(nturns), int(turn), int(npart),
+                                    float(energy), float(rest_energy), lam_ptr,
+                                    slots.ctypes.data_as(ip), coeffs.ctypes.data_as(ip),
+                                    grad_ptr, int(nparams), stream), self.lib)
+
     def check(self, energy: float, rest_energy: float) -> None:
         _check(self.lib.atx_lattice_check(self.h, float(energy), float(rest_energy)), self.lib)
 
@@ -247,12 +291,14 @@ class DeviceLattice:
 
 def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
                 rest_energy: float = 0.0, losses: bool = True, block_size: int = 0,
-                stream: int | None = None, strict: bool = False, soa: bool = False) -> None:
+                stream: int | None = None, strict: bool = False, soa: bool = False,
+                continue_losses: bool = False) -> None:
     """One launch over many (DeviceLattice, device particle buffer) pairs.
 
     items: sequence of dicts with keys ``lattice`` (DeviceLattice), ``r``
-    (device pointer), ``n`` (particles) and, with losses, ``lost`` = (flag,
-    turn, elem, coord) device pointers."""
+    (device pointer), ``n`` (particles), with losses ``lost`` = (flag,
+    turn, elem, coord) device pointers, and optionally ``nturns``/``turn``
+    for an item that tracks its own number of turns."""
     lib = load_library()
     arr = (BatchItem * max(len(items), 1))()
     for i, it in enumerate(items):
@@ -260,9 +306,12 @@ def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
         b.lattice = it["lattice"].h
         b.r = it["r"]
         b.num_particles = int(it["n"])
+        b.nturns = int(it.get("nturns", -1))
+        b.turn = int(it.get("turn", 0))
         if losses:
             b.lost_flag, b.lost_turn, b.lost_elem, b.lost_coord = it["lost"]
-    flags = (ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
+    flags = ((ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
+             | (ATX_CONTINUE_LOSSES if continue_losses else 0))
     _check(lib.atx_track_batch(arr, len(items), int(nturns), int(turn), float(energy),
                                float(rest_energy), int(bool(losses)), int(block_size), stream,
                                flags), lib)

@@ -1,0 +1,326 @@
+"""Continuous-batching tracking server.
+
+Lattice instances (error realisations of a ring) are compiled and uploaded
+once and stay resident on the device. Requests (a particle set, an
+instance, a number of turns) arrive over time. The server advances every
+admitted request by up to ``turn_block`` turns per launch, all of them in
+one ``atx_track_batch`` launch with per-item turn counts and turn numbers;
+between launches it retires requests whose turns are done and admits
+queued ones while the HBM budget and the slot limit allow. A request's
+particles and loss map stay on the device from admission to completion, and
+its loss map continues from launch to launch (ATX_CONTINUE_LOSSES), so the
+result equals one ``atx_track`` call over all its turns, bit for bit.
+
+Launches are queued ahead: the host knows from the turn counts which
+requests a launch completes, so it queues the next launch (and the copies
+back of finished requests, on a second stream) without waiting for the
+device; at most ``depth`` launches are in flight. A request is done when
+its results are in host memory; that time is read from a device event, so
+latencies do not include host polling delays.
+
+``serve_one_by_one`` is the baseline: requests in arrival order, one
+``atx_track`` launch each over all turns.
+
+No counterpart in the reference, whose tracking call is synchronous and
+per lattice (at.c:340-681).
+"""
+
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine
+
+BYTES_PER_PARTICLE = 48 + 1 + 4 + 4 + 48  # state + loss map (flag, turn, elem, coord)
+
+
+@dataclass
+class Request:
+    rid: int
+    instance: int             # index into the server's lattice instances
+    particles: np.ndarray     # (6, N) float64
+    nturns: int
+    arrival: float = 0.0      # seconds after the start of the trace
+    kind: str = "track"       # "track" or "da": both return coordinates and loss map
+
+    @property
+    def npart(self) -> int:
+        return int(self.particles.shape[1])
+
+
+@dataclass
+class Result:
+    rid: int
+    particles: np.ndarray     # (6, N) after nturns
+    lost: dict                # islost, turn, elem, coord as in lattice_track's loss map
+    arrival: float
+    admitted: float
+    done: float
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclass
+class _Slot:
+    req: Request
+    r: object                 # torch (N, 6) float64
+    flag: object
+    turn: object
+    elem: object
+    coord: object
+    done_turns: int = 0
+    admitted: float = 0.0
+    host: dict = field(default_factory=dict)
+
+
+def _pinned(shape, dtype):
+    import torch
+
+    return torch.empty(shape, dtype=dtype, pin_memory=True)
+
+
+class Server:
+    """Continuous batching over device-resident lattice instances.
+
+    lattices: sequence of element sequences (one per instance).
+    hbm_budget: bytes of particle state and loss maps admitted at once.
+    max_active: most requests in one launch.
+    turn_block: turns per launch (admission granularity).
+    depth: launches queued ahead of the device."""
+
+    def __init__(self, lattices, *, energy: float, rest_energy: float,
+                 hbm_budget: int = 64 << 30, max_active: int = 256, turn_block: int = 128,
+                 block_size: int = 0, depth: int = 2, strict: bool = False):
+        import torch
+
+        self.dls = [l if isinstance(l, engine.DeviceLattice) else engine.DeviceLattice(l)
+                    for l in lattices]
+        self.energy = float(energy)
+        self.rest = float(rest_energy)
+        self.budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.turn_block = int(turn_block)
+        self.block_size = int(block_size)
+        self.depth = max(1, int(depth))
+        self.strict = bool(strict)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.Stream(device=self.device)      # launches
+        self.copy_stream = torch.cuda.Stream(device=self.device)  # device-to-host results
+        for dl in self.dls:  # compile and upload every instance now, not on first use
+            dl.check(self.energy, self.rest)
+        self.launches = 0
+
+    def need(self, req: Request) -> int:
+        return req.npart * BYTES_PER_PARTICLE
+
+    def _admit(self, req: Request, now: float) -> _Slot:
+        import torch
+
+        n = req.npart
+        dev = self.device
+        host = _pinned((n, 6), torch.float64)
+        host.numpy()[:] = np.asarray(req.particles).T
+        with torch.cuda.stream(self.stream):
+            r = torch.empty((n, 6), dtype=torch.float64, device=dev)
+            r.copy_(host, non_blocking=True)
+            s = _Slot(req, r, torch.zeros(n, dtype=torch.uint8, device=dev),
+                      torch.zeros(n, dtype=torch.int32, device=dev),
+                      torch.zeros(n, dtype=torch.int32, device=dev),
+                      torch.zeros((n, 6), dtype=torch.float64, device=dev), admitted=now)
+        s.host["in"] = host  # keep the pinned source alive until the copy ran
+        return s
+
+    def _launch(self, active):
+        items = []
+        for s in active:
+            nt = min(self.turn_block, s.req.nturns - s.done_turns)
+            items.append({"lattice": self.dls[s.req.instance], "r": s.r.data_ptr(),
+                          "n": s.req.npart, "nturns": nt, "turn": s.done_turns,
+                          "lost": (s.flag.data_ptr(), s.turn.data_ptr(), s.elem.data_ptr(),
+                                   s.coord.data_ptr())})
+            s.done_turns += nt
+        engine.track_batch(items, 0, energy=self.energy, rest_energy=self.rest, losses=True,
+                           block_size=self.block_size, stream=self.stream.cuda_stream,
+                           strict=self.strict, continue_losses=True)
+        self.launches += 1
+
+    def _copy_back(self, s: _Slot):
+        import torch
+
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        self.copy_stream.wait_event(ev)
+        with torch.cuda.stream(self.copy_stream):
+            out = {k: _pinned(t.shape, t.dtype) for k, t in
+                   (("r", s.r), ("flag", s.flag), ("turn", s.turn), ("elem", s.elem),
+                    ("coord", s.coord))}
+            for k, t in (("r", s.r), ("flag", s.flag), ("turn", s.turn), ("elem", s.elem),
+                         ("coord", s.coord)):
+                out[k].copy_(t, non_blocking=True)
+                t.record_stream(self.copy_stream)
+            done = torch.cuda.Event(enable_timing=True)
+            done.record(self.copy_stream)
+        s.host.update(out)
+        return ev, done
+
+    def run(self, requests, realtime: bool = True):
+        """Serve ``requests`` (arrival times relative to the call, when
+        ``realtime``; otherwise all are queued at once). Returns results in
+        request order and a stats dict."""
+        import torch
+
+        pending = collections.deque(sorted(requests, key=lambda q: (q.arrival, q.rid)))
+        queue = collections.deque()
+        active: list[_Slot] = []
+        finishing = collections.deque()   # (slot, finish event, copy-done event)
+        inflight = collections.deque()    # launch-end events
+        results = {}
+        used = 0
+        torch.cuda.synchronize()
+        t0_ev = torch.cuda.Event(enable_timing=True)
+        t0_ev.record(self.stream)
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        while pending or queue or active or finishing:
+            tnow = now()
+            while pending and (not realtime or pending[0].arrival <= tnow):
+                queue.append(pending.popleft())
+            while (queue and len(active) < self.max_active
+                   and used + self.need(queue[0]) <= self.budget):
+                q = queue.popleft()
+                used += self.need(q)
+                active.append(self._admit(q, max(tnow, q.arrival if realtime else 0.0)))
+            if queue and not active and used == 0 and not finishing:
+                raise ValueError(f"request {queue[0].rid} needs {self.need(queue[0])} bytes, "
+                                 f"over the HBM budget of {self.budget}")
+            launched = False
+            if active and len(inflight) < self.depth:
+                self._launch(active)
+                launched = True
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+                inflight.append(ev)
+                still = []
+                for s in active:
+                    if s.done_turns >= s.req.nturns:
+                        fin, cp = self._copy_back(s)
+                        finishing.append((s, fin, cp))
+                    else:
+                        still.append(s)
+                active = still
+            # retire finished requests whose results are on the host
+            while finishing and finishing[0][2].query():
+                s, _, cp = finishing.popleft()
+                done = t0_ev.elapsed_time(cp) / 1000.0  # results on the host
+                h = s.host
+                results[s.req.rid] = Result(
+                    s.req.rid, h["r"].numpy().T.copy(),
+                    {"islost": h["flag"].numpy().astype(bool),
+                     "turn": h["turn"].numpy().view(np.uint32).copy(),
+                     "elem": h["elem"].numpy().view(np.uint32).copy(),
+                     "coord": h["coord"].numpy().T.copy()},
+                    s.req.arrival if realtime else 0.0, s.admitted, done)
+                used -= self.need(s.req)
+            while inflight and inflight[0].query():
+                inflight.popleft()
+            if launched:
+                continue
+            # nothing launched: wait for the device or for the next arrival
+            if inflight and (len(inflight) >= self.depth or not (queue or pending)):
+                inflight.popleft().synchronize()
+            elif finishing:
+                finishing[0][2].synchronize()
+            elif pending and not queue:
+                time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+        torch.cuda.synchronize()
+        wall = now()
+        res = [results[q.rid] for q in requests]
+        return res, _stats(res, wall, self.energy, self.launches)
+
+
+def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realtime: bool = True,
+                     strict: bool = False):
+    """Baseline: requests in arrival order, one atx_track launch each over
+    all of its turns, results copied back before the next one starts."""
+    import torch
+
+    dls = [l if isinstance(l, engine.DeviceLattice) else engine.DeviceLattice(l) for l in dls]
+    for dl in dls:
+        dl.check(energy, rest_energy)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    stream = torch.cuda.Stream(device=dev)
+    order = sorted(requests, key=lambda q: (q.arrival, q.rid))
+    results = {}
+    torch.cuda.synchronize()
+    t0_ev = torch.cuda.Event(enable_timing=True)
+    t0_ev.record(stream)
+    t0 = time.perf_counter()
+    for q in order:
+        if realtime:
+            wait = q.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        admitted = time.perf_counter() - t0
+        n = q.npart
+        with torch.cuda.stream(stream):
+            host = _pinned((n, 6), torch.float64)
+            host.numpy()[:] = np.asarray(q.particles).T
+            r = host.to(dev, non_blocking=True)
+            lf = torch.empty(n, dtype=torch.uint8, device=dev)
+            lt = torch.empty(n, dtype=torch.int32, device=dev)
+            le = torch.empty(n, dtype=torch.int32, device=dev)
+            lc = torch.empty((n, 6), dtype=torch.float64, device=dev)
+            dls[q.instance].track_device(r.data_ptr(), n, q.nturns, energy=energy,
+                                         rest_energy=rest_energy, losses=True,
+                                         loss_ptrs=(lf.data_ptr(), lt.data_ptr(), le.data_ptr(),
+                                                    lc.data_ptr()),
+                                         stream=stream.cuda_stream, strict=strict)
+            out = [_pinned(t.shape, t.dtype) for t in (r, lf, lt, le, lc)]
+            for o, t in zip(out, (r, lf, lt, le, lc)):
+                o.copy_(t, non_blocking=True)
+            fin = torch.cuda.Event(enable_timing=True)
+            fin.record(stream)  # results on the host
+        stream.synchronize()
+        done = t0_ev.elapsed_time(fin) / 1000.0
+        results[q.rid] = Result(q.rid, out[0].numpy().T.copy(),
+                                {"islost": out[1].numpy().astype(bool),
+                                 "turn": out[2].numpy().view(np.uint32).copy(),
+                                 "elem": out[3].numpy().view(np.uint32).copy(),
+                                 "coord": out[4].numpy().T.copy()},
+                                q.arrival if realtime else 0.0, admitted, done)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = [results[q.rid] for q in requests]
+    return res, _stats(res, wall, energy, len(order))
+
+
+def _stats(results, wall: float, energy: float, launches: int) -> dict:
+    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    return {"requests": len(results), "wall_s": wall, "launches": launches,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def alive_passes(requests, results, nelem_of) -> int:
+    """Element passes of surviving particles over all requests (the tracked
+    work, the throughput numerator): a particle lost at (turn t, element e)
+    contributes t*nelem + e + 1 passes."""
+    total = 0
+    for q, r in zip(requests, results):
+        ne = nelem_of(q.instance)
+        lost = r.lost["islost"]
+        total += int((~lost).sum()) * q.nturns * ne
+        if lost.any():
+            t = r.lost["turn"][lost].astype(np.int64)
+            e = r.lost["elem"][lost].astype(np.int64)
+            total += int((t * ne + e + 1).sum())
+    return total

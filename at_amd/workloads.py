@@ -203,3 +203,29 @@ def fodo_ring(ncells: int = 32, energy: float = 3.0e9) -> Lattice:
             ThinMultipole("SD", [0.0, 0.0, 0.0], [0.0, 0.0, -2.0]),
         ]
     return Lattice(elems, energy=energy, particle=Particle("electron"), name="fodo32")
+
+
+def serving_trace(n_requests: int = 32, rate: float = 100.0, seed: int = 3, n_instances: int = 64,
+                  nmin: int = 2000, nmax: int = 200000, turns=(256, 1024, 4096),
+                  da_fraction: float = 0.5):
+    """Request trace of a tracking service: Poisson arrivals at ``rate``
+    per second, an error-realisation index in [0, n_instances), N
+    log-uniform on [nmin, nmax], a turn count drawn from ``turns``; a
+    fraction ``da_fraction`` are dynamic-aperture grids (da_polar_grid), the
+    rest Gaussian beams. Returns a list of at_amd.serving.Request."""
+    from .serving import Request
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        inst = int(rng.integers(0, n_instances))
+        nt = int(rng.choice(np.asarray(turns)))
+        if rng.uniform() < da_fraction:
+            parts, kind = da_polar_grid(n), "da"
+        else:
+            parts, kind = gaussian_beam(n, seed=seed * 100003 + i), "track"
+        out.append(Request(i, inst, parts, nt, arrival=t, kind=kind))
+    return out

@@ -16,6 +16,7 @@ on the launch stream. cpu_baseline runs the reference integrator cores
 bounded sample of the same workload on the host cores.
 
     python bench.py [--steps 10] [--warmup 1] [--turns-per-step 100]
+    python bench.py --mode batch|serve|adjoint   (the other workloads, one line each)
 """
 
 from __future__ import annotations
@@ -56,9 +57,250 @@ def cpu_baseline(ring, beam_np, nturns: int, threads: int):
     }
 
 
+def _events(n):
+    import torch
+
+    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(n)]
+
+
+def bench_batch(args) -> dict:
+    """Batched error realisations: ``--instances`` error-seeded copies of the
+    large ring, each with a DA polar grid of U[1,000, 40,000] particles, all
+    tracked for ``--batch-turns`` turns in ONE launch (workgroup size from
+    the autotuner), against the same instances tracked one launch each."""
+    import numpy as np
+    import torch
+
+    from at_amd import engine
+    from at_amd.autotune import tune_batch
+    from at_amd.batch import track_batch
+    from at_amd.flops import tracked_flops_and_passes
+    from at_amd.workloads import batched_da_workload
+
+    dev = torch.device("cuda:0")
+    work = batched_da_workload(ninst=args.instances, seed=7)
+    lats = [l for l, _ in work]
+    dls = [engine.DeviceLattice(l) for l in lats]
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    base = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for _, p in work]
+    for dl in dls:
+        dl.check(energy, rest)
+    tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False)
+    block = tune["block_size"]
+    T = args.batch_turns
+    stream = torch.cuda.current_stream()
+    parts = [b.clone() for b in base]
+    (e0, e1), = _events(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    lms = track_batch(dls, parts, T, energy=energy, rest_energy=rest, losses=True,
+                      block_size=block)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    flops = passes = 0.0
+    for lat, lm in zip(lats, lms):
+        fl, pa = tracked_flops_and_passes(lat, T, 0, lm.flag.cpu().numpy().astype(bool),
+                                          lm.turn.cpu().numpy().view(np.uint32),
+                                          lm.elem.cpu().numpy().view(np.uint32))
+        flops += fl
+        passes += pa
+    seq_ms = None
+    if not args.no_compare:
+        parts2 = [b.clone() for b in base]
+        (s0, s1), = _events(1)
+        torch.cuda.synchronize()
+        s0.record(stream)
+        for dl, p in zip(dls, parts2):
+            dl.track_device(p.data_ptr(), p.shape[0], T, energy=energy, rest_energy=rest,
+                            stream=stream.cuda_stream)
+        s1.record(stream)
+        torch.cuda.synchronize()
+        seq_ms = s0.elapsed_time(s1)
+        same = all(torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0))
+                   for a, b in zip(parts, parts2))
+    achieved = flops / (ms * 1e-3) / 1e12
+    ntot = sum(int(b.shape[0]) for b in base)
+    return {
+        "metric": "batched error realisations: particle-element passes/s (one launch)",
+        "value": passes / (ms * 1e-3),
+        "unit": "particle-element passes/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(dls)} error realisations of large_ring ({len(lats[0])} "
+                               f"elements), DA polar grids of U[1000, 40000] particles "
+                               f"({ntot} total), {T} turns, one launch",
+                   "block_size": block, "autotune_ms": tune["ms"], "wall_s": wall},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+        "one_launch_per_instance_ms": seq_ms,
+        "speedup_vs_one_launch_per_instance": (seq_ms / ms) if seq_ms else None,
+        "identical_to_one_launch_per_instance": same if seq_ms else None,
+    }
+
+
+def bench_serve(args) -> dict:
+    """Served mode: a Poisson trace of ``--requests`` tracking and DA requests
+    over ``--instances`` error realisations of the large ring, N log-uniform
+    on [2e3, 2e5], turns in {256, 1024, 4096}, served by continuous batching
+    and by one launch per request."""
+    import numpy as np
+    import torch
+
+    from at_amd import engine
+    from at_amd.serving import Server, alive_passes, serve_one_by_one
+    from at_amd.workloads import error_realisation, large_ring, serving_trace
+
+    base = large_ring()
+    trace = serving_trace(n_requests=args.requests, rate=args.rate, n_instances=args.instances)
+    used = sorted({q.instance for q in trace})
+    lats = {i: error_realisation(base, 1000 + i) for i in used}
+    dls = {i: engine.DeviceLattice(lats[i]) for i in used}
+    # remap instance indices onto the compiled subset
+    idx = {i: k for k, i in enumerate(used)}
+    for q in trace:
+        q.instance = idx[q.instance]
+    dl_list = [dls[i] for i in used]
+    energy, rest = base.energy, base.particle.rest_energy
+    srv = Server(dl_list, energy=energy, rest_energy=rest, turn_block=args.turn_block)
+    # warm-up: one small request through each path
+    from at_amd.serving import Request
+    warm = [Request(0, 0, trace[0].particles[:, :64], 2)]
+    srv.run(warm, realtime=False)
+    serve_one_by_one(dl_list, warm, energy=energy, rest_energy=rest, realtime=False)
+    torch.cuda.synchronize()
+    res_b, st_b = srv.run(trace)
+    nel = len(base)
+    passes = alive_passes(trace, res_b, lambda i: nel)
+    out = {
+        "metric": "served tracking: particle-element passes/s, p50/p99 latency",
+        "value": passes / st_b["wall_s"],
+        "unit": "particle-element passes/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": st_b["wall_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(trace)} requests, Poisson {args.rate}/s, "
+                               f"{len(used)} error realisations of large_ring ({nel} elements), "
+                               f"N log-uniform [2e3, 2e5], turns in {{256, 1024, 4096}}",
+                   "turn_block": args.turn_block,
+                   "particles": int(sum(q.npart for q in trace)),
+                   "requests": [[q.npart, q.nturns, q.kind] for q in trace]},
+        "batched": st_b,
+    }
+    if not args.no_compare:
+        res_1, st_1 = serve_one_by_one(dl_list, trace, energy=energy, rest_energy=rest)
+        same = all(np.array_equal(a.particles, b.particles, equal_nan=True)
+                   and np.array_equal(a.lost["turn"], b.lost["turn"])
+                   and np.array_equal(a.lost["elem"], b.lost["elem"])
+                   for a, b in zip(res_b, res_1))
+        out["one_launch_per_request"] = st_1
+        out["one_launch_per_request"]["value"] = passes / st_1["wall_s"]
+        out["throughput_ratio"] = st_1["wall_s"] / st_b["wall_s"]
+        out["identical_results"] = bool(same)
+    return out
+
+
+def bench_adjoint(args) -> dict:
+    """Differentiation: gradient of L = sum(x^2+px^2+y^2+py^2) after 64
+    turns of the large ring (radiation off, cavity on) over 4,096 Gaussian
+    particles, with respect to PolynomB[1] of every quadrupole and
+    PolynomB[2] of every sextupole, checkpoints every 8 turns."""
+    import numpy as np
+    import torch
+
+    from at_amd import lattice as L
+    from at_amd.adjoint import Problem, TrackingLoss
+    from at_amd.workloads import gaussian_beam, large_ring
+
+    ring = large_ring(radiation=False, cavity=True)
+    params = []
+    for i, e in enumerate(ring):
+        if e.PassMethod.startswith("StrMPole") and e.MaxOrder in (1, 2):
+            params.append((i, e.MaxOrder))
+    prob = Problem(ring, params, checkpoint=8)
+    n, T = args.adj_particles, 64
+    beam = gaussian_beam(n, seed=5)
+    dev = torch.device("cuda:0")
+    parts = torch.from_numpy(np.ascontiguousarray(beam.T)).to(dev)
+    theta0 = prob.initial_strengths()
+    stream = torch.cuda.current_stream()
+
+    def one():
+        theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        loss = TrackingLoss.apply(prob, theta, parts, T)
+        loss.backward()
+        return loss, theta.grad
+
+    one()
+    torch.cuda.synchronize()
+    K = args.adj_steps
+    ev = _events(K)
+    fw = _events(K)
+    t0 = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        fw[k][0].record(stream)
+        loss = TrackingLoss.apply(prob, theta, parts, T)
+        fw[k][1].record(stream)
+        loss.backward()
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    fms = float(np.mean([a.elapsed_time(b) for a, b in fw]))
+    out = {
+        "metric": "strength gradients: gradient evaluations/s (forward + adjoint)",
+        "value": 1e3 / ms,
+        "unit": "gradients/s",
+        "n_gpus": 1, "steps": K, "warmup": 1, "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"large_ring radiation off, cavity on ({len(ring)} elements), "
+                               f"{n} particles, {T} turns, {len(params)} strengths "
+                               f"(PolynomB[1] of quadrupoles, PolynomB[2] of sextupoles), "
+                               f"checkpoints every 8 turns",
+                   "forward_ms": fms, "gradient_ms": ms,
+                   "adjoint_over_forward": (ms - fms) / fms},
+    }
+    if not args.no_cpu:
+        from oracle import pyoracle
+
+        impl = "ref" if pyoracle.available("ref") else "oracle"
+        threads = min(16, len(os.sched_getaffinity(0)))
+        m = min(n, args.cpu_particles)
+        r = np.asfortranarray(beam[:, :m].copy())
+        t0 = time.perf_counter()
+        pyoracle.atpass(ring, r, T, impl=impl, nthreads=threads)
+        dt = (time.perf_counter() - t0) * (n / m)
+        out["cpu_baseline"] = {
+            "value": 1.0 / (2 * len(params) * dt), "unit": "gradients/s", "cores": threads,
+            "kind": "reference" if impl == "ref" else "port",
+            "sample": f"central differences need 2 x {len(params)} trackings of {n} particles x "
+                      f"{T} turns; one tracking of {m} particles timed ({dt:.2f} s scaled to {n})",
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint"), default="track",
+                    help="track (default): the large-ring tracking line; batch: many error "
+                         "realisations in one launch; serve: continuous batching of a request "
+                         "trace; adjoint: strength gradients")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--batch-turns", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--turn-block", type=int, default=128)
+    ap.add_argument("--adj-particles", type=int, default=4096)
+    ap.add_argument("--adj-steps", type=int, default=3)
+    ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--turns-per-step", type=int, default=100)
@@ -96,6 +338,10 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     engine.set_device(0)
+    if args.mode != "track":
+        fn = {"batch": bench_batch, "serve": bench_serve, "adjoint": bench_adjoint}[args.mode]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
     ring = large_ring()
     nelem = len(ring)
     n = args.particles

@@ -41,6 +41,9 @@ extern "C" {
 #define ATX_STRICT 0x2u          /* reference operation order, no FMA contraction */
 #define ATX_KEEP_COUNTER 0x4u    /* continue from the turn counter saved by the previous call */
 #define ATX_SOA 0x8u             /* with ATX_DEVICE_POINTERS: r_in is SoA (x[N], px[N], ...) */
+#define ATX_CONTINUE_LOSSES 0x10u /* atx_track_batch: the loss arrays hold the map of earlier
+                                     calls on the same particles; they are not reset, and
+                                     particles already flagged are not recorded again */
 
 typedef struct atx_lattice atx_lattice;
 
@@ -107,13 +110,15 @@ int atx_elempass(atx_lattice *lat, int32_t index, double *r_in, uint32_t num_par
 
 /* --- Batched instances ---------------------------------------------------
  * Many lattice instances (e.g. error realisations), each with its own
- * device-resident particles, tracked for the same number of turns in ONE
- * launch: workgroups are assigned to instances in proportion to their
- * particle counts. No counterpart in the reference, which tracks one
- * lattice per call (at.c:340, PyInterface.cpp:132); each instance has the
- * semantics of atx_track with refpts = [] and start/end = whole ring.
- * All pointers are device pointers; the call is asynchronous on stream.
- * block_size: workgroup size (64..512, multiple of 64; 0 = 256). */
+ * device-resident particles, tracked in ONE launch: workgroups are assigned
+ * to instances in proportion to their particle counts, and the instances
+ * with the most turns x elements per workgroup are dispatched first. Each
+ * item tracks the call's nturns from the call's turn, or its own
+ * (nturns, turn) when item.nturns >= 0. No counterpart in the reference,
+ * which tracks one lattice per call (at.c:340, PyInterface.cpp:132); each
+ * instance has the semantics of atx_track with refpts = [] and start/end =
+ * whole ring. All pointers are device pointers; the call is asynchronous on
+ * stream. block_size: workgroup size (64..512, multiple of 64; 0 = 256). */
 typedef struct atx_batch_item {
     atx_lattice *lattice;
     double *r;              /* (6,N) F-order, or SoA with ATX_SOA */
@@ -123,11 +128,45 @@ typedef struct atx_batch_item {
     uint32_t *lost_turn;    /* N */
     uint32_t *lost_elem;    /* N */
     double *lost_coord;     /* (6,N) */
+    int32_t nturns;         /* turns of this item; < 0: the call's nturns and turn */
+    int32_t turn;           /* first turn number of this item when nturns >= 0 */
 } atx_batch_item;
 
 int atx_track_batch(const atx_batch_item *items, int32_t num_items, int32_t nturns, int32_t turn,
                     double energy, double rest_energy, int32_t losses, int32_t block_size,
                     void *stream, uint32_t flags);
+
+/* --- Differentiation -----------------------------------------------------
+ * Reverse-mode gradient through nturns turns of the whole ring. No
+ * counterpart in the reference, whose tangent maps (diff_drift.c,
+ * diff_thinkick.c, diff_bnd_kick.c, diff_bend_fringe.c) only propagate the
+ * diffusion matrix (at.c:755-831); this entry applies the transposes of the
+ * same per-step Jacobians.
+ *   states   device, (nturns+1) consecutive (6,N) F-order blocks: the state
+ *            at the start of each turn (as atx_track writes them with
+ *            refpts = [0]) followed by the state after the last turn;
+ *   lam      device (6,N): dL/d(end state) in, dL/d(start state) out;
+ *            lanes whose end state is lost are skipped;
+ *   grad_slot/grad_coeff  host arrays of nelem ints: element e contributes
+ *            dL/dPolynomB[grad_coeff[e]] to grad[grad_slot[e]] (slot -1:
+ *            not differentiated);
+ *   grad     device array of num_params doubles, accumulated (+=).
+ * Supported: IdentityPass, DriftPass, AperturePass, CorrectorPass,
+ * RFCavityPass, ThinMPolePass and the non-radiating Str/BndMPole passes
+ * with integrator 4, FieldScaling 1, no quadrupole fringe, edge fringe
+ * without gap correction, and misalignments that leave delta unchanged;
+ * anything else is ATX_E_RUNTIME. Asynchronous on stream. */
+int atx_adjoint(atx_lattice *lat, const double *states, int32_t nturns, int32_t turn,
+                uint32_t num_particles, double energy, double rest_energy, double *lam,
+                const int32_t *grad_slot, const int32_t *grad_coeff, double *grad,
+                int32_t num_params, void *stream);
+
+/* Replace an attribute of element `index` (e.g. PolynomB of a magnet whose
+ * strength is a differentiation parameter); the compiled lattice is
+ * rebuilt on the next call. No counterpart in the reference, which
+ * re-reads the Python elements unless reuse is set (at.c:445-512). */
+int atx_element_set_array_at(atx_lattice *lat, int32_t index, const char *name,
+                             const double *data, int64_t n);
 
 /* Turn counter after the last call (at.c:656 last_turn). */
 int atx_lattice_turn_counter(const atx_lattice *lat, int64_t *turn);

@@ -211,3 +211,24 @@ def elempass(element, rin, energy=0.0, particle=None, impl="oracle"):
     if rc:
         raise (ValueError if rc in (1, 3) else RuntimeError)(lib.err().decode())
     return rin
+
+
+def tangent_ref(element, r6) -> np.ndarray:
+    """6x6 Jacobian (row-major) of one Str/BndMPole element at r6, read off
+    the reference's own diffusion-matrix propagation (diff_drift.c,
+    diff_str_kick.c, diff_bnd_kick.c, diff_bend_fringe.c) through
+    ref_tangent in ref_driver.c. Elements must have no T1/T2, R1/R2 or
+    quadrupole fringe, which the cores do not propagate."""
+    lib = _lib("ref")
+    fn = lib.lib.ref_tangent
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(OracleElem), _dp, _dp]
+    keep: list = []
+    s = make_elem(element, keep)
+    r = np.ascontiguousarray(r6, dtype=np.float64).reshape(6)
+    m = np.zeros((6, 6), np.float64)
+    rc = fn(C.byref(s), _ptr(r), _ptr(m))
+    if rc:
+        raise RuntimeError(lib.err().decode())
+    del keep
+    return m

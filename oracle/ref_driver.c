@@ -323,6 +323,59 @@ int ref_atpass(const oracle_elem *elems, int32_t nelem, double *r_in, int32_t np
     return 0;
 }
 
+/*
+ * Tangent map of one multipole element at one particle, from the reference
+ * itself: the radiating cores (StrMPoleSymplectic4RadPass.c:83-91,
+ * BndMPoleSymplectic4RadPass.c:86-108), given a diffusion-matrix pointer,
+ * apply B <- M B M^T with the per-step Jacobians M of diff_drift.c,
+ * diff_str_kick.c / diff_bnd_kick.c (thinkickM) and diff_bend_fringe.c.
+ * With gamma = 1e-9 the radiation and diffusion terms are below FP64
+ * resolution, so B_out = M B_in M^T with M the Jacobian of the
+ * non-radiating body. Starting from B_in = e_j e_4^T gives
+ * B_out = (M e_j)(M e_4)^T, and since M[4][4] = 1 the column 4 of B_out
+ * is column j of M. Six runs fill m66 (row-major). T1/T2, R1/R2 and the
+ * quadrupole fringes are not propagated by the cores: use elements
+ * without them. r6 is left unchanged.
+ */
+int ref_tangent(const oracle_elem *e, const double *r6, double *m66)
+{
+    rf_err[0] = 0;
+    oracle_elem rad = *e;
+    if (e->pass == OR_STRMPOLESYMPLECTIC4PASS || e->pass == OR_STRMPOLESYMPLECTIC4RADPASS)
+        rad.pass = OR_STRMPOLESYMPLECTIC4RADPASS;
+    else if (e->pass == OR_BNDMPOLESYMPLECTIC4PASS || e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS)
+        rad.pass = OR_BNDMPOLESYMPLECTIC4RADPASS;
+    else
+        return rf_fail(4, "tangent maps exist for the multipole passes only");
+    void *f = rf_resolve(rad.pass);
+    if (!f) return rf_fail(4, rf_err);
+    for (int j = 0; j < 6; j++) {
+        rf_prep p;
+        int rc = rf_prepare(&rad, &p, 1.0e-9, 1.0, 0.0); /* gamma = 1e-9 */
+        if (rc) return rc;
+        double r[6], bdiff[36];
+        memcpy(r, r6, sizeof(r));
+        memset(bdiff, 0, sizeof(bdiff));
+        bdiff[j + 6 * 4] = 1.0;
+        if (rad.pass == OR_STRMPOLESYMPLECTIC4RADPASS)
+            ((fn_strrad)f)(r, e->Length, p.a, p.b, e->MaxOrder, e->NumIntSteps,
+                           e->FringeQuadEntrance, e->FringeQuadExit, D(e->fringeIntM0),
+                           D(e->fringeIntP0), D(e->T1), D(e->T2), D(e->R1), D(e->R2),
+                           D(e->RApertures), D(e->EApertures), D(e->KickAngle), e->FieldScaling,
+                           p.gamma, 1, bdiff);
+        else
+            ((fn_bndrad)f)(r, e->Length, p.irho, p.a, p.b, e->MaxOrder, e->NumIntSteps,
+                           e->EntranceAngle, e->ExitAngle, e->FringeBendEntrance,
+                           e->FringeBendExit, e->FringeInt1, e->FringeInt2, e->FullGap,
+                           e->FringeQuadEntrance, e->FringeQuadExit, D(e->fringeIntM0),
+                           D(e->fringeIntP0), D(e->T1), D(e->T2), D(e->R1), D(e->R2),
+                           D(e->RApertures), D(e->EApertures), D(e->KickAngle), e->FieldScaling,
+                           p.gamma, 1, bdiff);
+        for (int i = 0; i < 6; i++) m66[6 * i + j] = bdiff[i + 6 * 4];
+    }
+    return 0;
+}
+
 int ref_elempass(const oracle_elem *e, double *r_in, int32_t np, double energy,
                  double rest_energy)
 {

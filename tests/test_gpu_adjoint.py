@@ -1,0 +1,230 @@
+"""GPU: the adjoint kernel (atx_adjoint) and the autograd wrapper.
+
+* Per-element tangent maps: with lambda = e_i on particle i, one adjoint
+  pass returns row i of the element's Jacobian. It must equal the
+  reference's own diff_* Jacobian (read off its diffusion-matrix
+  propagation, oracle/ref_driver.c:ref_tangent) and central finite
+  differences of the oracle.
+* Strength gradients of L = sum_alive (x^2+px^2+y^2+py^2) after 64 turns of
+  the large ring (radiation off, cavity on), with respect to PolynomB[1] of
+  every quadrupole and PolynomB[2] of every sextupole, against central
+  finite differences (h = 1e-7*|K|) of the compiled reference tracking.
+* Size-independent checks at the full 4,096-particle size: the gradient is
+  additive over particle subsets, and independent of the checkpoint spacing.
+"""
+
+import copy
+import os
+
+import numpy as np
+import pytest
+
+from at_amd import engine
+from at_amd import lattice as L
+from at_amd.adjoint import Problem, loss_and_grad
+from at_amd.workloads import gaussian_beam, large_ring
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ENERGY = 6.0e9
+THREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+def _one(elem):
+    return L.Lattice([elem], energy=ENERGY, particle="electron")
+
+
+def _device_jacobian(elem, points):
+    """Rows of the element Jacobian at each point, through atx_adjoint."""
+    import torch
+
+    lat = _one(elem)
+    dl = engine.DeviceLattice(lat)
+    npnt = len(points)
+    n = 6 * npnt
+    dev = torch.device("cuda:0")
+    start = torch.tensor(np.repeat(np.asarray(points), 6, axis=0), dtype=torch.float64, device=dev)
+    states = torch.empty((2, n, 6), dtype=torch.float64, device=dev)
+    states[0].copy_(start)
+    states[1].copy_(start)
+    stream = torch.cuda.current_stream().cuda_stream
+    rest = lat.particle.rest_energy
+    dl.track_device(states[1].data_ptr(), n, 1, energy=ENERGY, rest_energy=rest, stream=stream)
+    lam = torch.eye(6, dtype=torch.float64, device=dev).repeat(npnt, 1).contiguous()
+    dl.adjoint(states.data_ptr(), 1, 0, n, ENERGY, rest, lam.data_ptr(),
+               np.full(1, -1, np.int32), np.zeros(1, np.int32), 0, 0, stream=stream)
+    torch.cuda.synchronize()
+    return lam.cpu().numpy().reshape(npnt, 6, 6)
+
+
+def _fd(elem, r6, h=1e-7):
+    lat = _one(elem)
+
+    def f(r):
+        a = np.asfortranarray(r.reshape(6, 1).copy())
+        po.atpass(lat, a, 1)
+        return a[:, 0]
+
+    jac = np.zeros((6, 6))
+    for j in range(6):
+        d = np.zeros(6)
+        d[j] = h
+        jac[:, j] = (f(r6 + d) - f(r6 - d)) / (2 * h)
+    return jac
+
+
+POINTS = [np.array([1e-3, -2e-4, 5e-4, 1e-4, 2e-3, 0.0]),
+          np.array([-3e-3, 4e-4, -1e-3, -2e-4, -0.02, 1e-3]),
+          np.zeros(6)]
+
+
+def _multipoles():
+    return {
+        "quad": L.Quadrupole("Q", 0.5, 1.3, NumIntSteps=10),
+        "defoc_quad": L.Quadrupole("QD", 0.31, -2.1, NumIntSteps=7),
+        "sext": L.Sextupole("S", 0.2, 35.0, NumIntSteps=8),
+        "oct": L.Octupole("O", 0.1, [], [0.0, 0.0, 0.0, 500.0], NumIntSteps=4),
+        "skew": L.Multipole("M", 0.3, [0.0, 0.4, 2.0], [0.0, 0.7, -3.0], NumIntSteps=6),
+        "bend": L.Dipole("B", 1.0, 0.03, 0.0, EntranceAngle=0.015, ExitAngle=0.015),
+        "cf_bend": L.Dipole("BQ", 1.0669994258, 0.0292, -0.9, EntranceAngle=0.0146,
+                            ExitAngle=0.0146, NumIntSteps=10),
+        "bend_m0": L.Dipole("B0", 0.8, 0.02, 0.2, EntranceAngle=0.01, ExitAngle=0.0,
+                            FringeBendEntrance=0, FringeBendExit=0),
+    }
+
+
+@pytest.mark.skipif(not po.available("ref"), reason="oracle/_ref not built")
+@pytest.mark.parametrize("name", list(_multipoles()))
+def test_tangent_maps_equal_reference_diff_maps(name):
+    elem = _multipoles()[name]
+    rows = _device_jacobian(elem, POINTS)
+    for ip, r6 in enumerate(POINTS):
+        m_ref = po.tangent_ref(elem, r6)
+        scale = max(1.0, np.abs(m_ref).max())
+        err = np.abs(rows[ip] - m_ref).max()
+        print(f"{name} point {ip}: max |M_adj - M_ref| = {err:.3e}", flush=True)
+        assert err <= 1e-12 * scale, rows[ip] - m_ref
+
+
+def _others():
+    return {
+        "drift": L.Drift("D", 2.7),
+        "thin": L.ThinMultipole("T", [0.0, 0.1], [0.0, 1.5, 20.0]),
+        "corrector": L.Corrector("C", 0.1, [1e-4, -2e-4]),
+        "rf": L.RFCavity("RF", 0.0, 6e6, 352e6 * 1.0, 992, ENERGY, TimeLag=0.05),
+        "misaligned_quad": L.Quadrupole("QM", 0.4, 1.1, T1=np.array([1e-4, 0, -2e-4, 0, 0, 0]),
+                                        T2=np.array([-1e-4, 0, 2e-4, 0, 0, 0]),
+                                        R1=_rot(1e-3), R2=_rot(-1e-3)),
+        "method2_bend": L.Dipole("B2", 1.0, 0.05, 0.0, EntranceAngle=0.025, ExitAngle=0.025,
+                                 FringeBendEntrance=2, FringeBendExit=2),
+        "kicked_bend": L.Dipole("BK", 1.0, 0.03, 0.1, KickAngle=np.array([1e-4, 5e-5])),
+    }
+
+
+def _rot(psi):
+    c, s = np.cos(psi), np.sin(psi)
+    r = np.eye(6)
+    r[0, 0] = r[1, 1] = r[2, 2] = r[3, 3] = c
+    r[0, 2] = r[1, 3] = s
+    r[2, 0] = r[3, 1] = -s
+    return r
+
+
+@pytest.mark.parametrize("name", list(_others()) + list(_multipoles()))
+def test_tangent_maps_equal_fd_of_oracle(name):
+    elem = {**_others(), **_multipoles()}[name]
+    rows = _device_jacobian(elem, POINTS)
+    for ip, r6 in enumerate(POINTS):
+        jac = _fd(elem, r6)
+        scale = max(1.0, np.abs(jac).max())
+        err = np.abs(rows[ip] - jac).max()
+        print(f"{name} point {ip}: max |M_adj - M_fd| = {err:.3e}", flush=True)
+        assert err <= 2e-8 * scale, rows[ip] - jac
+
+
+def _ring_problem(checkpoint=8):
+    ring = large_ring(radiation=False, cavity=True)
+    params = []
+    for i, e in enumerate(ring):
+        if isinstance(e, L.Quadrupole) or (e.PassMethod.startswith("StrMPole") and e.MaxOrder == 1):
+            params.append((i, 1))
+        elif e.PassMethod.startswith("StrMPole") and e.MaxOrder == 2:
+            params.append((i, 2))
+    return ring, Problem(ring, params, checkpoint=checkpoint)
+
+
+def _loss_cpu(ring, beam, nturns, edits, impl):
+    lat = ring.copy()
+    for idx, k, v in edits:
+        e = copy.copy(lat[idx])
+        pb = np.array(e.PolynomB, dtype=np.float64)
+        pb[k] = v
+        e.PolynomB = pb
+        lat[idx] = e
+    r = beam.copy(order="F")
+    po.atpass(lat, r, nturns, impl=impl, nthreads=THREADS)
+    alive = np.all(np.isfinite(r), axis=0)
+    q = r[:4, alive]
+    return float(np.sum(q * q))
+
+
+@pytest.mark.skipif(not po.available("ref"), reason="oracle/_ref not built")
+def test_strength_gradients_equal_fd_of_reference_tracking():
+    ring, prob = _ring_problem()
+    nq = sum(1 for _, k in prob.params if k == 1)
+    ns = sum(1 for _, k in prob.params if k == 2)
+    print(f"{len(ring)} elements, {nq} quadrupole and {ns} sextupole strengths", flush=True)
+    beam = gaussian_beam(128, seed=11)
+    nturns = 64
+    loss, grad = loss_and_grad(prob, beam, nturns)
+    theta = prob.initial_strengths()
+    l_ref = _loss_cpu(ring, beam, nturns, [], "ref")
+    print(f"loss gpu {loss!r} reference {l_ref!r}", flush=True)
+    assert abs(loss - l_ref) <= 1e-9 * abs(l_ref)
+    # FD on a spread of parameters: first/last quadrupole and sextupole of
+    # several families, in different cells
+    sext = [i for i, (_, k) in enumerate(prob.params) if k == 2]
+    picks = sorted(set([0, 7, len(theta) // 2, len(theta) - 1, sext[0], sext[3], sext[-1]]))
+    gmax = np.abs(grad).max()
+    for i in picks:
+        idx, k = prob.params[i]
+        h = 1e-7 * abs(theta[i])
+        lp = _loss_cpu(ring, beam, nturns, [(idx, k, theta[i] + h)], "ref")
+        lm = _loss_cpu(ring, beam, nturns, [(idx, k, theta[i] - h)], "ref")
+        g_fd = (lp - lm) / (2 * h)
+        print(f"param {i} elem {idx} PolynomB[{k}]={theta[i]:.6g}: adjoint {grad[i]:.9e} "
+              f"fd {g_fd:.9e}", flush=True)
+        # FD rounding: eps*L/h relative to the strength ~ 1e-16*L/(1e-7*|K|)
+        tol = 1e-6 * abs(g_fd) + 1e-15 * l_ref / h + 1e-9 * gmax
+        assert abs(grad[i] - g_fd) <= tol
+
+
+def test_gradient_additive_over_particles_and_checkpoint_independent():
+    ring, prob8 = _ring_problem(checkpoint=8)
+    beam = gaussian_beam(4096, seed=5)
+    nturns = 64
+    loss, grad = loss_and_grad(prob8, beam, nturns)
+    parts = [loss_and_grad(prob8, np.asfortranarray(beam[:, s:s + 1024]), nturns)
+             for s in range(0, 4096, 1024)]
+    lsum = sum(p[0] for p in parts)
+    gsum = sum(p[1] for p in parts)
+    gmax = np.abs(grad).max()
+    print(f"4096 particles: loss {loss!r}, max |grad| {gmax:.3e}", flush=True)
+    assert abs(lsum - loss) <= 1e-12 * abs(loss)
+    assert np.abs(gsum - grad).max() <= 1e-11 * gmax
+    _, prob1 = _ring_problem(checkpoint=64)
+    _, g1 = loss_and_grad(prob1, np.asfortranarray(beam[:, :512]), nturns)
+    _, prob3 = _ring_problem(checkpoint=3)
+    _, g3 = loss_and_grad(prob3, np.asfortranarray(beam[:, :512]), nturns)
+    _, g8 = loss_and_grad(prob8, np.asfortranarray(beam[:, :512]), nturns)
+    assert np.abs(g1 - g8).max() <= 1e-11 * np.abs(g8).max()
+    assert np.abs(g3 - g8).max() <= 1e-11 * np.abs(g8).max()
+
+
+def test_adjoint_rejects_radiating_lattice():
+    ring = large_ring(radiation=True, cavity=True, ncells=1)
+    idx = next(i for i, e in enumerate(ring) if isinstance(e, L.Quadrupole))
+    prob = Problem(ring, [(idx, 1)])
+    with pytest.raises(RuntimeError, match="adjoint not available"):
+        loss_and_grad(prob, gaussian_beam(64, seed=1), 2)

@@ -1,0 +1,137 @@
+"""GPU: continuous batching (at_amd.serving).
+
+A request served in the mix (advanced a block of turns per batched launch,
+admitted while others run, loss map continued across launches) must give
+exactly what the same request gives alone in one atx_track launch over all
+its turns: coordinates, loss flags, loss turns, loss elements and loss
+coordinates, bit for bit. Per-item turn counts of atx_track_batch and the
+continued loss map are checked against single launches too."""
+
+import numpy as np
+import pytest
+
+from at_amd import engine
+from at_amd.batch import LossMap, track_batch
+from at_amd.serving import Request, Server, serve_one_by_one
+from at_amd.workloads import da_polar_grid, error_realisation, gaussian_beam, large_ring
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(n=3, ncells=2):
+    base = large_ring(ncells=ncells, cavity=True)
+    return [error_realisation(base, 500 + i) for i in range(n)]
+
+
+def _alone(dl, q, energy, rest, strict=False):
+    r = np.asfortranarray(q.particles.copy())
+    rout, lm = dl.track(r, q.nturns, None, energy=energy, rest_energy=rest, losses=True,
+                        strict=strict)
+    return r, lm
+
+
+def _trace(nreq=10, seed=2):
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(nreq):
+        t += float(rng.exponential(0.01))
+        n = int(rng.integers(50, 3000))
+        nt = int(rng.choice([5, 17, 40]))
+        parts = da_polar_grid(n, xmax=12e-3, ymax=5e-3) if i % 2 else gaussian_beam(n, seed=i, scale=20.0)
+        out.append(Request(i, int(rng.integers(0, 3)), parts, nt, arrival=t,
+                           kind="da" if i % 2 else "track"))
+    return out
+
+
+@pytest.mark.parametrize("realtime", [False, True], ids=["queued", "arrivals"])
+def test_served_results_equal_alone(realtime):
+    lats = _instances()
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dls = [engine.DeviceLattice(l) for l in lats]
+    trace = _trace()
+    srv = Server(dls, energy=energy, rest_energy=rest, turn_block=8, max_active=4,
+                 hbm_budget=6000 * 105)
+    res, st = srv.run(trace, realtime=realtime)
+    print(st, flush=True)
+    assert st["launches"] > 1
+    nlost = 0
+    for q, r in zip(trace, res):
+        r_alone, lm = _alone(dls[q.instance], q, energy, rest)
+        np.testing.assert_array_equal(r.particles, r_alone)
+        np.testing.assert_array_equal(r.lost["islost"], lm["islost"])
+        np.testing.assert_array_equal(r.lost["turn"], lm["turn"])
+        np.testing.assert_array_equal(r.lost["elem"], lm["elem"])
+        np.testing.assert_array_equal(r.lost["coord"], lm["coord"])
+        nlost += int(lm["islost"].sum())
+        assert r.done >= r.admitted - 1e-3
+    assert nlost > 0  # the trace exercises the continued loss map
+
+
+def test_one_by_one_baseline_equals_served():
+    lats = _instances()
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dls = [engine.DeviceLattice(l) for l in lats]
+    trace = _trace(6, seed=4)
+    res_b, _ = Server(dls, energy=energy, rest_energy=rest, turn_block=16).run(trace, realtime=False)
+    res_1, _ = serve_one_by_one(dls, trace, energy=energy, rest_energy=rest, realtime=False)
+    for a, b in zip(res_b, res_1):
+        np.testing.assert_array_equal(a.particles, b.particles)
+        np.testing.assert_array_equal(a.lost["turn"], b.lost["turn"])
+        np.testing.assert_array_equal(a.lost["elem"], b.lost["elem"])
+
+
+def test_strict_served_request_equals_oracle():
+    # without the cavity (whose sin is the only device transcendental) the
+    # strict build and the oracle agree bit for bit
+    lat = error_realisation(large_ring(ncells=2, cavity=False), 77)
+    energy, rest = lat.energy, lat.particle.rest_energy
+    dl = engine.DeviceLattice(lat)
+    q = Request(0, 0, da_polar_grid(500, xmax=12e-3, ymax=5e-3), 9)
+    (res,), _ = Server([dl], energy=energy, rest_energy=rest, turn_block=4,
+                       strict=True).run([q], realtime=False)
+    ref = np.asfortranarray(q.particles.copy())
+    _, lmo = po.atpass(lat, ref, 9, losses=True)
+    assert lmo["islost"].any()
+    np.testing.assert_array_equal(res.lost["islost"], lmo["islost"])
+    np.testing.assert_array_equal(res.lost["turn"], lmo["turn"])
+    np.testing.assert_array_equal(res.lost["elem"], lmo["elem"])
+    np.testing.assert_array_equal(res.lost["coord"], lmo["coord"])
+    np.testing.assert_array_equal(res.particles, ref)
+
+
+def test_batch_item_turns_and_continued_losses():
+    import torch
+
+    lats = _instances(2)
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dls = [engine.DeviceLattice(l) for l in lats]
+    dev = torch.device("cuda:0")
+    parts = [da_polar_grid(700, xmax=12e-3, ymax=5e-3), da_polar_grid(333, xmax=12e-3, ymax=5e-3)]
+    tens = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    lms = [LossMap.empty(p.shape[1], dev) for p in parts]
+    # instance 0: turns 0-2 then 3-9; instance 1: turns 0-6 then nothing
+    track_batch(dls, tens, 0, energy=energy, rest_energy=rest, loss_maps=lms,
+                item_turns=[(3, 0), (7, 0)], continue_losses=True)
+    track_batch(dls, tens, 0, energy=energy, rest_energy=rest, loss_maps=lms,
+                item_turns=[(7, 3), (0, 7)], continue_losses=True)
+    torch.cuda.synchronize()
+    for i, nt in ((0, 10), (1, 7)):
+        r = np.asfortranarray(parts[i].copy())
+        _, lm = dls[i].track(r, nt, None, energy=energy, rest_energy=rest, losses=True)
+        np.testing.assert_array_equal(tens[i].cpu().numpy().T, r)
+        np.testing.assert_array_equal(lms[i].flag.cpu().numpy().astype(bool), lm["islost"])
+        np.testing.assert_array_equal(lms[i].turn.cpu().numpy().view(np.uint32), lm["turn"])
+        np.testing.assert_array_equal(lms[i].elem.cpu().numpy().view(np.uint32), lm["elem"])
+        np.testing.assert_array_equal(lms[i].coord.cpu().numpy().T, lm["coord"])
+        assert lm["islost"].any()
+
+
+def test_budget_too_small_is_an_error():
+    lats = _instances(1)
+    dl = engine.DeviceLattice(lats[0])
+    srv = Server([dl], energy=lats[0].energy, rest_energy=lats[0].particle.rest_energy,
+                 hbm_budget=1000)
+    with pytest.raises(ValueError, match="HBM budget"):
+        srv.run([Request(0, 0, gaussian_beam(100), 2)], realtime=False)
