@@ -207,21 +207,9 @@ __device__ __forceinline__ void rad_drift(R6& r, double len) {
 }
 
 #if !ATX_STRICT
-// Fast build: inside a radiating magnet delta changes only in the kicks, by
-// the radiated fraction (1e-8 or less per kick in storage rings), so
-// pn = 1/(1+delta) is carried from step to step and refreshed after each
-// kick by two Newton steps from the previous value,
-// pn' = pn (1 + e), e = 1 - (1+delta') pn: the error after the two steps is
-// e^4 plus rounding, within an ulp for |e| < 1e-4.
-__device__ __forceinline__ double newton_pn(double pn, double opd) {
-  double e = __builtin_fma(-opd, pn, 1.0);
-  pn = __builtin_fma(pn, e, pn);
-  e = __builtin_fma(-opd, pn, 1.0);
-  return __builtin_fma(pn, e, pn);
-}
-
-// 1/d from the hardware reciprocal estimate and two Newton steps (the
-// estimate's relative error, below 2^-20, is squared twice).
+// Fast build: 1/d from the hardware reciprocal estimate (v_rcp_f64) and two
+// Newton steps q <- q + q (1 - d q), each of which squares the relative
+// error of the estimate; the result is within an ulp of 1/d.
 __device__ __forceinline__ double fast_rcp(double d) {
   double q = __builtin_amdgcn_rcp(d);
   double e = __builtin_fma(-d, q, 1.0);
@@ -230,7 +218,9 @@ __device__ __forceinline__ double fast_rcp(double d) {
   return __builtin_fma(q, e, q);
 }
 
-// Radiating drift with the carried pn.
+// Fast build: inside a radiating magnet delta changes only in the kicks, so
+// pn = 1/(1+delta) is computed once at entry and then by each kick for the
+// radiated delta, and carried to the drifts.
 __device__ __forceinline__ void rad_drift_pn(R6& r, double len, double pn) {
   const double nl = len * pn;
   r.x = __builtin_fma(nl, r.px, r.x);
@@ -277,7 +267,7 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
   const double opd2 = 1.0 + r.dp;
   r.px = __builtin_fma(xpr, opd2, -kl * re);
   r.py = __builtin_fma(ypr, opd2, kl * im);
-  pn_io = newton_pn(pn_in, opd2);
+  pn_io = fast_rcp(opd2);
 #endif
 }
 
@@ -334,7 +324,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   r.px = __builtin_fma(xpr, opd2, -kl * v);
   r.py = __builtin_fma(ypr, opd2, kl * im);
   r.ct = __builtin_fma(kli, x, r.ct);
-  pn_io = newton_pn(pn_in, opd2);
+  pn_io = fast_rcp(opd2);
 #endif
 }
 
@@ -727,6 +717,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
     store6(I.lost_coord + 6 * (int64_t)p, 1, z);
   }
   const int nturns = I.nturns;
+  const bool refs = (I.n_first | I.n_std) != 0;  // uniform: skip the slot reads without refpts
   for (int t = 0; t < nturns; t++) {
     const int nturn = I.turn0 + t;
     const int e0 = (t == 0) ? I.start_elem : 0;
@@ -734,8 +725,10 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
     const int64_t base = (t == 0) ? 0 : (int64_t)I.n_first + (int64_t)(t - 1) * I.n_std;
     const int e1 = I.end_elem;
     for (int e = e0; e < e1; e++) {
-      const int sl = slot[e];
-      if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+      if (refs) {
+        const int sl = slot[e];
+        if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+      }
       cDevElem& h = elems[e];
       if (!__builtin_isnan(r.x)) element(r, h, pool, nturn);
       if (check) {
@@ -753,8 +746,10 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
         }
       }
     }
-    const int sl = slot[nelem];
-    if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+    if (refs) {
+      const int sl = slot[nelem];
+      if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+    }
   }
   rp[0] = r.x;
   rp[cs] = r.px;

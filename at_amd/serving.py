@@ -169,10 +169,11 @@ class Server:
         s.host.update(out)
         return ev, done
 
-    def run(self, requests, realtime: bool = True):
+    def run(self, requests, realtime: bool = True, log=None):
         """Serve ``requests`` (arrival times relative to the call, when
         ``realtime``; otherwise all are queued at once). Returns results in
-        request order and a stats dict."""
+        request order and a stats dict. ``log(result)`` is called as each
+        request completes."""
         import torch
 
         pending = collections.deque(sorted(requests, key=lambda q: (q.arrival, q.rid)))
@@ -230,6 +231,8 @@ class Server:
                      "coord": h["coord"].numpy().T.copy()},
                     s.req.arrival if realtime else 0.0, s.admitted, done)
                 used -= self.need(s.req)
+                if log is not None:
+                    log(results[s.req.rid])
             while inflight and inflight[0].query():
                 inflight.popleft()
             if launched:
@@ -248,7 +251,7 @@ class Server:
 
 
 def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realtime: bool = True,
-                     strict: bool = False):
+                     strict: bool = False, log=None):
     """Baseline: requests in arrival order, one atx_track launch each over
     all of its turns, results copied back before the next one starts."""
     import torch
@@ -297,6 +300,8 @@ def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realti
                                  "elem": out[3].numpy().view(np.uint32).copy(),
                                  "coord": out[4].numpy().T.copy()},
                                 q.arrival if realtime else 0.0, admitted, done)
+        if log is not None:
+            log(results[q.rid])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     res = [results[q.rid] for q in requests]

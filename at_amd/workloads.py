@@ -172,8 +172,16 @@ def da_polar_grid(n: int, xmax: float = 15e-3, ymax: float = 6e-3,
     return np.asfortranarray(np.concatenate(cols, axis=1))
 
 
+# Transverse shift of quadrupoles and sextupoles in the error realisations
+# that are tracked for many turns: the uncorrected large ring has no closed
+# orbit with 30 um rms shifts (every particle is lost within 3 turns), so
+# these workloads use 3 um, the residual level after orbit correction; rolls
+# (100 urad) and gradient errors (1e-3) keep their full size.
+RESIDUAL_SHIFT = 3e-6
+
+
 def batched_da_workload(ninst: int = 64, seed: int = 7, nmin: int = 1000, nmax: int = 40000,
-                        ncells: int | None = None):
+                        ncells: int | None = None, shift: float = RESIDUAL_SHIFT):
     """Error-seeded copies of the large ring with DA grids of U[nmin, nmax]
     particles each: a list of (lattice, particles)."""
     rng = np.random.default_rng(seed)
@@ -181,7 +189,7 @@ def batched_da_workload(ninst: int = 64, seed: int = 7, nmin: int = 1000, nmax: 
     out = []
     for i in range(ninst):
         n = int(rng.integers(nmin, nmax + 1))
-        out.append((error_realisation(base, seed * 1000 + i), da_polar_grid(n)))
+        out.append((error_realisation(base, seed * 1000 + i, shift=shift), da_polar_grid(n)))
     return out
 
 

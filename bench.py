@@ -132,7 +132,8 @@ def bench_batch(args) -> dict:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(dls)} error realisations of large_ring ({len(lats[0])} "
-                               f"elements), DA polar grids of U[1000, 40000] particles "
+                               f"elements; rolls 100 urad, gradients 1e-3, residual shifts "
+                               f"3 um), DA polar grids of U[1000, 40000] particles "
                                f"({ntot} total), {T} turns, one launch",
                    "block_size": block, "autotune_ms": tune["ms"], "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
@@ -153,12 +154,12 @@ def bench_serve(args) -> dict:
 
     from at_amd import engine
     from at_amd.serving import Server, alive_passes, serve_one_by_one
-    from at_amd.workloads import error_realisation, large_ring, serving_trace
+    from at_amd.workloads import RESIDUAL_SHIFT, error_realisation, large_ring, serving_trace
 
     base = large_ring()
     trace = serving_trace(n_requests=args.requests, rate=args.rate, n_instances=args.instances)
     used = sorted({q.instance for q in trace})
-    lats = {i: error_realisation(base, 1000 + i) for i in used}
+    lats = {i: error_realisation(base, 1000 + i, shift=RESIDUAL_SHIFT) for i in used}
     dls = {i: engine.DeviceLattice(lats[i]) for i in used}
     # remap instance indices onto the compiled subset
     idx = {i: k for k, i in enumerate(used)}
@@ -173,7 +174,12 @@ def bench_serve(args) -> dict:
     srv.run(warm, realtime=False)
     serve_one_by_one(dl_list, warm, energy=energy, rest_energy=rest, realtime=False)
     torch.cuda.synchronize()
-    res_b, st_b = srv.run(trace)
+
+    def log(r):
+        print(f"request {r.rid} done at {r.done:.2f} s (latency {r.latency:.2f} s)",
+              file=sys.stderr, flush=True)
+
+    res_b, st_b = srv.run(trace, log=log)
     nel = len(base)
     passes = alive_passes(trace, res_b, lambda i: nel)
     out = {
@@ -184,7 +190,8 @@ def bench_serve(args) -> dict:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(trace)} requests, Poisson {args.rate}/s, "
-                               f"{len(used)} error realisations of large_ring ({nel} elements), "
+                               f"{len(used)} error realisations of large_ring ({nel} elements; "
+                               f"rolls 100 urad, gradients 1e-3, residual shifts 3 um), "
                                f"N log-uniform [2e3, 2e5], turns in {{256, 1024, 4096}}",
                    "turn_block": args.turn_block,
                    "particles": int(sum(q.npart for q in trace)),
@@ -192,7 +199,7 @@ def bench_serve(args) -> dict:
         "batched": st_b,
     }
     if not args.no_compare:
-        res_1, st_1 = serve_one_by_one(dl_list, trace, energy=energy, rest_energy=rest)
+        res_1, st_1 = serve_one_by_one(dl_list, trace, energy=energy, rest_energy=rest, log=log)
         same = all(np.array_equal(a.particles, b.particles, equal_nan=True)
                    and np.array_equal(a.lost["turn"], b.lost["turn"])
                    and np.array_equal(a.lost["elem"], b.lost["elem"])
