@@ -70,7 +70,8 @@ enum Slot : int32_t {
   S_EDGE_IN = 12, S_CORR_IN = 13, S_EDGE_OUT = 14, S_CORR_OUT = 15,
   S_RADC = 16,
   S_BAX = 17, S_BAY = 18,
-  S_NSCALAR = 20,
+  S_RK1 = 19, S_RK2 = 20,                   // S_RADC * S_K1, S_RADC * S_K2 (fast build)
+  S_NSCALAR = 22,
   // RF cavity
   S_RF_HALF = 0, S_RF_NV = 1, S_RF_TWOPIF = 2, S_RF_LAG = 3, S_RF_PHILAG = 4, S_RF_HFT0 = 5,
   // corrector

@@ -473,6 +473,8 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
         }
         const double gamma = (rest == 0.0) ? 1.0E-9 * energy / kE0GeV : energy / rest;
         sc[S_RADC] = kRadConst * std::pow(gamma, 3);
+        sc[S_RK1] = sc[S_RADC] * sc[S_K1];
+        sc[S_RK2] = sc[S_RADC] * sc[S_K2];
       }
     } break;
   }

@@ -218,6 +218,14 @@ __device__ __forceinline__ double fast_rcp(double d) {
   return __builtin_fma(q, e, q);
 }
 
+// One Newton step: for a reciprocal that only scales the radiated energy (a
+// relative change of 1e-8 or less per kick), where an error of the squared
+// estimate error is far below the rounding of delta.
+__device__ __forceinline__ double rcp1(double d) {
+  const double q = __builtin_amdgcn_rcp(d);
+  return __builtin_fma(q, __builtin_fma(-d, q, 1.0), q);
+}
+
 // Fast build: inside a radiating magnet delta changes only in the kicks, so
 // pn = 1/(1+delta) is computed once at entry and then by each kick for the
 // radiated delta, and carried to the drifts.
@@ -231,15 +239,17 @@ __device__ __forceinline__ void rad_drift_pn(R6& r, double len, double pn) {
 #endif
 
 // Radiating straight kick (diff_str_kick.c:19-83, bdiff == NULL).
-// The fast build takes pn = 1/(1+delta) at entry and returns it updated to
-// the radiated delta.
+// radc = RAD_CONST*gamma^3 (strict); radk = radc*kl (fast). The fast build
+// takes pn = 1/(1+delta) at entry and returns it updated to the radiated
+// delta.
 template <int N>
 __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                             double radc, double& pn_io) {
+                                             double radc, double radk, double& pn_io) {
   double re, im;
   field<N>(r, a, b, n, re, im);
 #if ATX_STRICT
   (void)pn_io;
+  (void)radk;
   double pn = 1.0 / (1.0 + r.dp);
   const double xpr = r.px * pn;
   const double ypr = r.py * pn;
@@ -254,16 +264,16 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
   r.px -= kl * re;
   r.py += kl * im;
 #else
+  (void)radc;
   const double pn_in = pn_io;
   const double opd = 1.0 + r.dp;
   const double xpr = r.px * pn_in;
   const double ypr = r.py * pn_in;
   const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
-  const double vn2 = fast_rcp(1.0 + s2);
   const double cr = __builtin_fma(im, ypr, -re * xpr);
-  const double b2p = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im)) * vn2;
-  const double factor = __builtin_fma(0.5, s2, 1.0) * (opd * opd) * kl;
-  r.dp = __builtin_fma(-radc * b2p, factor, r.dp);
+  const double b2 = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im));
+  const double g = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);  // (1+s2/2)/(1+s2)
+  r.dp = __builtin_fma(-radk, (b2 * g) * (opd * opd), r.dp);
   const double opd2 = 1.0 + r.dp;
   r.px = __builtin_fma(xpr, opd2, -kl * re);
   r.py = __builtin_fma(ypr, opd2, kl * im);
@@ -274,12 +284,13 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
 // Radiating curved-frame kick (diff_bnd_kick.c:85-174, bdiff == NULL).
 template <int N>
 __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                             double irho, double kli, double radc,
+                                             double irho, double kli, double radc, double radk,
                                              double& pn_io) {
   double re, im;
   field<N>(r, a, b, n, re, im);
 #if ATX_STRICT
   (void)pn_io;
+  (void)radk;
   double pn = 1.0 / (1.0 + r.dp);
   const double x = r.x;
   const double xpr = r.px * pn;
@@ -302,6 +313,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   r.py += kl * im;
   r.ct += kli * r.x;
 #else
+  (void)radc;
   const double pn_in = pn_io;
   const double x = r.x;
   const double opd = 1.0 + r.dp;
@@ -310,14 +322,13 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   const double by = re + irho;
   const double hx = __builtin_fma(x, irho, 1.0);
   const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
-  const double vn2 = fast_rcp(__builtin_fma(hx, hx, s2));
   const double byh = by * hx;
   const double bxh = im * hx;
   const double cr = __builtin_fma(im, ypr, -by * xpr);
-  const double b2p = __builtin_fma(cr, cr, __builtin_fma(byh, byh, bxh * bxh)) * vn2;
-  const double factor = __builtin_fma(0.5, s2, hx) * (opd * opd) * kl;
+  const double b2 = __builtin_fma(cr, cr, __builtin_fma(byh, byh, bxh * bxh));
+  const double g = __builtin_fma(0.5, s2, hx) * rcp1(__builtin_fma(hx, hx, s2));
   const double d0 = r.dp;
-  r.dp = __builtin_fma(-radc * b2p, factor, r.dp);
+  r.dp = __builtin_fma(-radk, (b2 * g) * (opd * opd), r.dp);
   const double opd2 = 1.0 + r.dp;
   const double u = __builtin_fma(-x, irho, d0);
   const double v = __builtin_fma(-u, irho, re);
@@ -442,8 +453,8 @@ __device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdoub
       } else {
         if (RAD) {
           double pn = 1.0 / (1.0 + r.dp);
-          if (BEND) rad_bnd_kick<-1>(r, a, b, n, len, irho, len * irho, radc, pn);
-          else rad_str_kick<-1>(r, a, b, n, len, radc, pn);
+          if (BEND) rad_bnd_kick<-1>(r, a, b, n, len, irho, len * irho, radc, radc * len, pn);
+          else rad_str_kick<-1>(r, a, b, n, len, radc, radc * len, pn);
         } else {
           if (BEND) bnd_kick<-1>(r, a, b, n, len, irho, len * irho); else str_kick<-1>(r, a, b, n, len);
         }
@@ -519,28 +530,29 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
     double pn = 0.0;  // unused: the strict steps divide as the reference does
     for (int m = 0; m < h.nslice; m++) {
       rad_drift(r, l1);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, 0.0, pn);
       rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, k2, radc, 0.0, pn);
       rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, 0.0, pn);
       rad_drift(r, l1);
     }
 #else
+    const double rk1 = s[S_RK1], rk2 = s[S_RK2];
     double pn = 1.0 / (1.0 + r.dp);  // one division per magnet
     for (int m = 0; m < h.nslice; m++) {
       rad_drift_pn(r, l1, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, rk1, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, rk1, pn);
       rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k2, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, rk2, pn);
+      else rad_str_kick<N>(r, a, b, n, k2, radc, rk2, pn);
       rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, rk1, pn);
+      else rad_str_kick<N>(r, a, b, n, k1, radc, rk1, pn);
       rad_drift_pn(r, l1, pn);
     }
 #endif

@@ -31,6 +31,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= matrix) peak, TFLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "track_kernel_large_ring_traffic.json")
+
+
+def measured_traffic(particles: int, turns: int):
+    """HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs of the
+    same launch shape (copy-calibrated), or None when they were taken on
+    another shape."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("particles") != particles or t.get("turns_per_launch") != turns:
+        return None
+    return t.get("bytes_per_launch")
 METRIC = "particle-element passes/s FP64 + %FP64 peak; served p50/p99 latency vs plain"
 
 
@@ -440,7 +455,7 @@ def main() -> int:
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(n, T) if not args.strict else None,
             "kernel_ms_mean": mean_ms,
             "flops_per_launch": float(np.mean(flops_per_launch)),
         },
