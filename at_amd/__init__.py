@@ -11,3 +11,22 @@ from .lattice import (  # noqa: F401
     ThinMultipole, Multipole, Dipole, Bend, Quadrupole, Sextupole, Octupole, RFCavity,
     Corrector, Lattice, load_mat, e_mass, clight,
 )
+
+from . import acceptance as _acceptance  # noqa: E402
+from . import physics as _physics  # noqa: E402
+from .acceptance import (  # noqa: E402,F401
+    GridMode, get_1d_acceptance, get_acceptance, get_horizontal_acceptance,
+    get_momentum_acceptance, get_vertical_acceptance,
+)
+from .physics import (  # noqa: E402,F401
+    DConstant, ELossMethod, find_elem_m66, find_m44, find_m66, find_orbit, find_orbit4,
+    find_orbit6, find_sync_orbit, get_energy_loss, get_timelag_fromU0,
+)
+
+# Lattice methods, as pyat attaches them (physics/__init__.py, acceptance)
+for _name in ("find_orbit", "find_orbit4", "find_orbit6", "find_sync_orbit", "find_m44",
+              "find_m66", "get_energy_loss", "get_timelag_fromU0"):
+    setattr(Lattice, _name, getattr(_physics, _name))
+for _name in ("get_acceptance", "get_1d_acceptance", "get_horizontal_acceptance",
+              "get_vertical_acceptance", "get_momentum_acceptance"):
+    setattr(Lattice, _name, getattr(_acceptance, _name))

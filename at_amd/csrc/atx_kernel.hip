@@ -686,7 +686,28 @@ __device__ __forceinline__ void store6(double* base, int64_t cs, const R6& r) {
   base[5 * cs] = r.ct;
 }
 
+// Dynamic LDS of a workgroup of B lanes, used to compact surviving particles
+// into the leading waves: 6*B doubles of state, B ints of particle index
+// (bit 31: loss already recorded), 2 x 8 per-wave counts (double-buffered
+// by round, so that a wave can publish its next count while a slower wave
+// still reads the current ones).
+__host__ __device__ constexpr size_t compact_lds_bytes(int block) {
+  return (size_t)block * (6 * sizeof(double) + sizeof(int)) + 2 * 8 * sizeof(int);
+}
+
+__device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
+  double* rp = I.r + (int64_t)p * I.pstride;
+  const int64_t cs = I.cstride;
+  rp[0] = r.x;
+  rp[cs] = r.px;
+  rp[2 * cs] = r.y;
+  rp[3 * cs] = r.py;
+  rp[4 * cs] = r.dp;
+  rp[5 * cs] = r.ct;
+}
+
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
+  extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
   // Instance of this workgroup: the last whose first block is <= blockIdx.x.
   int lo = 0, hi = ninst - 1;
@@ -696,48 +717,120 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
     if (insts[mid].block_begin <= bid) lo = mid; else hi = mid - 1;
   }
   cInstance& I = insts[lo];
-  const int p = (bid - I.block_begin) * (int)blockDim.x + (int)threadIdx.x;
+  const int tid = (int)threadIdx.x;
+  const int B = (int)blockDim.x;
   const int npart = I.npart;
-  if (p >= npart) return;
+  const int first = (bid - I.block_begin) * B;  // first particle of this workgroup
+  // waves of this workgroup that hold particles (uniform); the others end now
+  int nw = (min(B, npart - first) + 63) >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (wave >= nw) return;
 
+  int p = first + tid;
+  bool has = p < npart;  // this lane holds a particle
   cDevElem* elems = (cDevElem*)I.elems;
   cdouble* pool = (cdouble*)I.pool;
   cint* slots = (cint*)I.slots;
-  double* rp = I.r + (int64_t)p * I.pstride;
   const int64_t cs = I.cstride;
-  R6 r;
-  r.x = rp[0];
-  r.px = rp[cs];
-  r.y = rp[2 * cs];
-  r.py = rp[3 * cs];
-  r.dp = rp[4 * cs];
-  r.ct = rp[5 * cs];
+  R6 r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (has) {
+    const double* rp = I.r + (int64_t)p * I.pstride;
+    r.x = rp[0];
+    r.px = rp[cs];
+    r.y = rp[2 * cs];
+    r.py = rp[3 * cs];
+    r.dp = rp[4 * cs];
+    r.ct = rp[5 * cs];
+  }
 
   const int nelem = I.nelem;
   const int losses = I.losses;
   const bool check = !I.elempass;
   double* rout = I.rout;
   const int64_t np6 = 6 * (int64_t)npart;
-  bool lost = false;
-  if (losses == 2) {
-    lost = I.lost_flag[p] != 0;  // recorded by an earlier call
-  } else if (losses) {
-    I.lost_flag[p] = 0;
-    I.lost_turn[p] = 0;
-    I.lost_elem[p] = 0;
-    R6 z = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    store6(I.lost_coord + 6 * (int64_t)p, 1, z);
+  bool lost = !has;  // lanes without a particle never record a loss
+  if (has) {
+    if (losses == 2) {
+      lost = I.lost_flag[p] != 0;  // recorded by an earlier call
+    } else if (losses) {
+      I.lost_flag[p] = 0;
+      I.lost_turn[p] = 0;
+      I.lost_elem[p] = 0;
+      R6 z = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      store6(I.lost_coord + 6 * (int64_t)p, 1, z);
+    }
   }
   const int nturns = I.nturns;
   const bool refs = (I.n_first | I.n_std) != 0;  // uniform: skip the slot reads without refpts
+  // Lost particles are dropped at turn boundaries and the survivors packed
+  // into the leading waves, so that waves left without particles end and
+  // free their SIMD slots. Not with refpts: lost particles still fill their
+  // refpt copies, as in at.c.
+  const bool compact = check && !refs && nturns > 1;
+  double* s_r = lds;
+  int* s_p = (int*)(lds + 6 * B);
+  int* s_cnt = s_p + B;
   for (int t = 0; t < nturns; t++) {
+    if (compact && t > 0) {
+      // A particle whose x is NaN is lost for good (elements skip it and the
+      // loss check leaves it as it is): store its final state now.
+      if (has && __builtin_isnan(r.x)) {
+        store_state(I, p, r);
+        has = false;
+      }
+      const uint64_t m = __ballot(has);
+      const int cnt = __popcll(m);
+      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+      int* cnt_buf = s_cnt + 8 * (t & 1);
+      if ((tid & 63) == 0) cnt_buf[wave] = cnt;
+      __syncthreads();
+      int off = 0, total = 0;
+      for (int k = 0; k < nw; k++) {
+        const int c = cnt_buf[k];
+        off += (k < wave) ? c : 0;
+        total += c;
+      }
+      total = __builtin_amdgcn_readfirstlane(total);
+      const int need = (total + 63) >> 6;
+      if (need < nw) {  // uniform: packing frees at least one wave
+        if (has) {
+          const int j = off + rank;
+          s_r[j] = r.x;
+          s_r[B + j] = r.px;
+          s_r[2 * B + j] = r.y;
+          s_r[3 * B + j] = r.py;
+          s_r[4 * B + j] = r.dp;
+          s_r[5 * B + j] = r.ct;
+          s_p[j] = p | (lost ? (int)0x80000000u : 0);
+        }
+        __syncthreads();
+        has = tid < total;
+        if (has) {
+          r.x = s_r[tid];
+          r.px = s_r[B + tid];
+          r.y = s_r[2 * B + tid];
+          r.py = s_r[3 * B + tid];
+          r.dp = s_r[4 * B + tid];
+          r.ct = s_r[5 * B + tid];
+          const int q = s_p[tid];
+          p = q & 0x7fffffff;
+          lost = q < 0;
+        } else {
+          r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
+          lost = true;
+        }
+        nw = need;
+        if (wave >= nw) return;  // this wave holds no particle any more
+      }
+    }
     const int nturn = I.turn0 + t;
     const int e0 = (t == 0) ? I.start_elem : 0;
     cint* slot = slots + ((t == 0) ? 0 : nelem + 1);
     const int64_t base = (t == 0) ? 0 : (int64_t)I.n_first + (int64_t)(t - 1) * I.n_std;
     const int e1 = I.end_elem;
     for (int e = e0; e < e1; e++) {
-      if (refs) {
+      if (refs && has) {
         const int sl = slot[e];
         if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
       }
@@ -758,17 +851,12 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
         }
       }
     }
-    if (refs) {
+    if (refs && has) {
       const int sl = slot[nelem];
       if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
     }
   }
-  rp[0] = r.x;
-  rp[cs] = r.px;
-  rp[2 * cs] = r.y;
-  rp[3 * cs] = r.py;
-  rp[4 * cs] = r.dp;
-  rp[5 * cs] = r.ct;
+  if (has) store_state(I, p, r);
 }
 
 }  // namespace ATX_NS
@@ -776,7 +864,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
                                  hipStream_t stream) {
-  hipLaunchKernelGGL(atx::ATX_NS::track_kernel, dim3(nblocks), dim3(block), 0, stream, d_insts,
-                     ninst);
+  hipLaunchKernelGGL(atx::ATX_NS::track_kernel, dim3(nblocks), dim3(block),
+                     atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
   return hipGetLastError();
 }

@@ -358,6 +358,13 @@ class Lattice(list):
     def deepcopy(self) -> "Lattice":
         return _copy.deepcopy(self)
 
+    def rotate(self, n: int) -> "Lattice":
+        """New lattice starting at element n (lattice_object.py:537-542)."""
+        if len(self) == 0:
+            return self.copy()
+        n = n % len(self)
+        return self._with_elems(list(self[n:]) + list(self[:n]))
+
     def _with_elems(self, elems) -> "Lattice":
         lat = _copy.copy(self)
         list.__init__(lat, elems)

@@ -1,0 +1,102 @@
+"""GPU: lost-particle compaction inside the tracking kernel.
+
+At every turn boundary of a call without refpts the kernel stores the final
+state of particles lost during the previous turn, packs the survivors of a
+workgroup into its leading waves and ends the waves left empty. Results must
+not change: the strict build stays bit-identical to the oracle (coordinates
+and loss map), for every workgroup size, with and without a loss map, and
+when the loss map continues across calls."""
+
+import numpy as np
+import pytest
+
+from at_amd import engine
+from at_amd.batch import LossMap, track_batch
+from at_amd.workloads import da_polar_grid, error_realisation, large_ring
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _ring():
+    # no cavity: the strict build then matches the oracle bit for bit
+    return error_realisation(large_ring(ncells=2, cavity=False), 11)
+
+
+def _grid(n=3001):
+    return da_polar_grid(n, xmax=14e-3, ymax=6e-3)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
+def test_compacted_batch_equals_oracle(block):
+    import torch
+
+    lat = _ring()
+    dl = engine.DeviceLattice(lat)
+    parts = _grid()
+    dev = torch.device("cuda:0")
+    t = torch.from_numpy(np.ascontiguousarray(parts.T)).to(dev)
+    (lm,) = track_batch([dl], [t], 12, energy=lat.energy, rest_energy=lat.particle.rest_energy,
+                        block_size=block, strict=True)
+    torch.cuda.synchronize()
+    ref = parts.copy(order="F")
+    _, lmo = po.atpass(lat, ref, 12, losses=True)
+    frac = lmo["islost"].mean()
+    print(f"block {block}: lost fraction {frac:.3f}", flush=True)
+    assert 0.2 < frac < 0.95
+    np.testing.assert_array_equal(t.cpu().numpy().T, ref)
+    np.testing.assert_array_equal(lm.flag.cpu().numpy().astype(bool), lmo["islost"])
+    np.testing.assert_array_equal(lm.turn.cpu().numpy().view(np.uint32), lmo["turn"])
+    np.testing.assert_array_equal(lm.elem.cpu().numpy().view(np.uint32), lmo["elem"])
+    np.testing.assert_array_equal(lm.coord.cpu().numpy().T, lmo["coord"])
+
+
+def test_compacted_track_without_loss_map_equals_oracle():
+    lat = _ring()
+    dl = engine.DeviceLattice(lat)
+    parts = _grid(777)
+    got = parts.copy(order="F")
+    dl.track(got, 9, None, energy=lat.energy, rest_energy=lat.particle.rest_energy,
+             strict=True)
+    ref = parts.copy(order="F")
+    po.atpass(lat, ref, 9)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_compacted_fast_build_keeps_loss_indices():
+    lat = _ring()
+    dl = engine.DeviceLattice(lat)
+    parts = _grid(2048)
+    got = parts.copy(order="F")
+    _, lm = dl.track(got, 12, None, energy=lat.energy, rest_energy=lat.particle.rest_energy,
+                     losses=True)
+    ref = parts.copy(order="F")
+    _, lmo = po.atpass(lat, ref, 12, losses=True)
+    np.testing.assert_array_equal(lm["islost"], lmo["islost"])
+    np.testing.assert_array_equal(lm["turn"], lmo["turn"])
+    np.testing.assert_array_equal(lm["elem"], lmo["elem"])
+    surv = ~lmo["islost"]
+    np.testing.assert_allclose(got[:, surv], ref[:, surv], rtol=0, atol=1e-11)
+    assert np.all(np.isnan(got[0, ~surv]))
+
+
+def test_compaction_with_continued_loss_map():
+    import torch
+
+    lat = _ring()
+    dl = engine.DeviceLattice(lat)
+    parts = _grid(1500)
+    dev = torch.device("cuda:0")
+    t = torch.from_numpy(np.ascontiguousarray(parts.T)).to(dev)
+    lm = LossMap.empty(parts.shape[1], dev)
+    e, rest = lat.energy, lat.particle.rest_energy
+    for t0, nt in ((0, 4), (4, 5), (9, 3)):
+        track_batch([dl], [t], 0, energy=e, rest_energy=rest, loss_maps=[lm],
+                    item_turns=[(nt, t0)], continue_losses=True, strict=True)
+    torch.cuda.synchronize()
+    ref = parts.copy(order="F")
+    _, lmo = po.atpass(lat, ref, 12, losses=True)
+    np.testing.assert_array_equal(t.cpu().numpy().T, ref)
+    np.testing.assert_array_equal(lm.flag.cpu().numpy().astype(bool), lmo["islost"])
+    np.testing.assert_array_equal(lm.turn.cpu().numpy().view(np.uint32), lmo["turn"])
+    np.testing.assert_array_equal(lm.elem.cpu().numpy().view(np.uint32), lmo["elem"])

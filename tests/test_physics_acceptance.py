@@ -1,0 +1,136 @@
+"""CPU: orbit, matrix and acceptance drivers against the reference's
+known answers (pyat/test/test_physics.py, test_acceptance.py), with the
+oracle doing the tracking (tests/oracle_hooks.py). The same drivers run on
+the engine in tests/test_gpu_physics_acceptance.py."""
+
+import os
+
+import numpy as np
+import pytest
+
+import at_amd
+from at_amd import load_mat, physics
+from oracle import pyoracle as po
+
+import oracle_hooks
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# pyat/test/test_physics.py:21-41
+ORBIT6_MATLAB = np.array([-2.6352679435692906e-09, -1.4584877344311391e-10, 0.0, 0.0,
+                          -6.6969442207863660e-06, -5.8845211247215173e-02])
+M66_MATLAB = np.array([
+    [-0.735631089984354, 4.673714021050702, 0.0, 0.0, 0.002998514873422, -0.000000627691677],
+    [-0.098166216409437, -0.735666318495682, 0.0, 0.0, 0.000169015216087, -0.000000035380663],
+    [0.0, 0.0, 0.609804485536081, -2.096029241592979, 0.0, 0.0],
+    [0.0, 0.0, 0.299675179703767, 0.609800209788848, 0.0, 0.0],
+    [0.000001246879424, 0.000021544877348, 0.0, 0.0, 0.999980690841359, -0.000209330146676],
+    [0.000170098657382, 0.002995796897591, 0.0, 0.0, 0.002243258562703, 0.999999530409042]])
+# pyat/test/test_physics.py:117-133
+ELEM19_M66 = np.array([[1.0386, 0.180911, 0.0, 0.0, 0.0, 0.0],
+                       [0.434959, 1.0386, 0.0, 0.0, 0.0, 0.0],
+                       [0.0, 0.0, 0.961891, 0.176344, 0.0, 0.0],
+                       [0.0, 0.0, -0.423978, 0.961891, 0.0, 0.0],
+                       [0.0, 0.0, 0.0, 0.0, 1.0, 0.0],
+                       [0.0, 0.0, 0.0, 0.0, 0.0, 1.0]])
+# pyat/test/test_acceptance.py:6-32
+ACC_2D = np.array([[-0.01125, -0.0053033, 0., 0.00574524, 0.010625],
+                   [0., 0.0053033, 0.006875, 0.00574524, 0.]])
+
+pytestmark = pytest.mark.skipif(not po.available("oracle"), reason="oracle not built")
+
+
+@pytest.fixture
+def hooks(monkeypatch):
+    oracle_hooks.install(monkeypatch)
+
+
+@pytest.fixture(scope="module")
+def hmba():
+    return load_mat(os.path.join(GOLDEN, "hmba.mat"))
+
+
+def test_find_orbit6_known_answer(hooks, hmba):
+    orbit6, _ = physics.find_orbit6(hmba.radiation_on(copy=True))
+    np.testing.assert_allclose(orbit6, ORBIT6_MATLAB, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("refpts", ([120], [20], [1, 2, 3]))
+def test_find_m66_known_answer(hooks, hmba, refpts):
+    m66, mstack = hmba.radiation_on(copy=True).find_m66(refpts=refpts)
+    np.testing.assert_allclose(m66, M66_MATLAB, rtol=0, atol=1e-8)
+    assert mstack.shape == (len(refpts), 6, 6)
+
+
+@pytest.mark.parametrize("index", (19, 0, 1))
+def test_find_elem_m66_known_answer(hooks, hmba, index):
+    m66 = physics.find_elem_m66(hmba[index])
+    expected = ELEM19_M66 if index == 19 else np.eye(6)
+    np.testing.assert_allclose(m66, expected, rtol=1e-5, atol=1e-7)
+
+
+def test_find_orbit4_zero_and_propagation(hooks, hmba):
+    ring = hmba.radiation_off(copy=True)
+    orbit4, _ = physics.find_orbit4(ring, 0)
+    np.testing.assert_allclose(orbit4, np.zeros(6), atol=1e-7)
+    # off momentum: the orbit is a fixed point of the one-turn map
+    o, pts = physics.find_orbit4(ring, 1e-4, [0, 60])
+    r = np.asfortranarray(o.reshape(6, 1).copy())
+    po.atpass(ring, r, 1, energy=ring.energy)
+    np.testing.assert_allclose(r[:4, 0], o[:4], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(pts[0], o, rtol=0, atol=0)
+    assert pts.shape == (2, 6)
+
+
+def test_orbit_max_iterations_warns(hooks, hmba):
+    with pytest.warns(at_amd.AtWarning):
+        physics.find_orbit4(hmba.radiation_off(copy=True), 1e-3, max_iterations=1)
+
+
+def test_find_m44_symplectic(hooks, hmba):
+    m44, ms = physics.find_m44(hmba.radiation_off(copy=True), dp=0.0, refpts=[5], full=True)
+    j = np.kron(np.eye(2), np.array([[0.0, 1.0], [-1.0, 0.0]]))
+    np.testing.assert_allclose(m44.T @ j @ m44, j, atol=1e-7)
+    assert ms.shape == (1, 4, 4)
+
+
+def test_1d_acceptance_known_answers(hooks, hmba):
+    ring = hmba.radiation_off(copy=True)
+    acc, _, _ = ring.get_horizontal_acceptance(1e-3, 1.0e-3)
+    np.testing.assert_allclose(acc, [-0.011, 0.011], atol=1e-6)
+    acc, _, _ = ring.get_horizontal_acceptance(1e-3, 1.0e-3, refpts=10)
+    np.testing.assert_allclose(acc, [-0.002, 0.002], atol=1e-6)
+
+
+def test_2d_recursive_acceptance_known_answer(hooks, hmba):
+    ring = hmba.radiation_off(copy=True)
+    acc, _, _ = ring.get_acceptance(["x", "y"], [10, 5], [10.0e-3, 10.0e-3],
+                                   grid_mode=at_amd.GridMode.RECURSIVE)
+    np.testing.assert_allclose(acc, ACC_2D, atol=1e-6, rtol=1.0e-4)
+
+
+def test_grid_modes_bracket_the_recursive_boundary(hooks, hmba):
+    """A RADIAL grid of the same angles finds, along each angle, the last
+    surviving grid radius: within one grid step of the recursive result."""
+    ring = hmba.radiation_off(copy=True)
+    rec, _, _ = ring.get_acceptance(["x", "y"], [10, 5], [10.0e-3, 10.0e-3],
+                                   grid_mode=at_amd.GridMode.RECURSIVE)
+    bnd, surv, grid = ring.get_acceptance(["x", "y"], [21, 5], [20.0e-3, 20.0e-3],
+                                          grid_mode=at_amd.GridMode.RADIAL, use_mp=True)
+    assert grid.shape == (2, 21 * 5) and surv.shape[0] == 2
+    r_rec = np.hypot(*rec)
+    r_grid = np.hypot(*bnd)
+    assert np.all(np.abs(r_grid - r_rec) <= 1e-3 + 1e-9)
+    cb, _, cg = ring.get_acceptance(["x", "y"], [11, 6], [15.0e-3, 8.0e-3],
+                                    grid_mode=at_amd.GridMode.CARTESIAN, use_mp=True)
+    assert cg.shape == (2, 66) and cb.shape[0] == 2
+
+
+def test_recursive_with_no_survivor_warns(hooks, hmba):
+    ring = hmba.radiation_off(copy=True)
+    with pytest.warns(at_amd.AtWarning):
+        bnd, surv, _ = ring.get_acceptance(["x", "y"], [5, 3], [0.5, 0.5],
+                                           grid_mode=at_amd.GridMode.RADIAL, use_mp=True,
+                                           bounds=[[0.5, 1.0], [np.pi, 0]], nturns=4)
+    assert surv.shape[1] == 0
+    np.testing.assert_array_equal(bnd, np.zeros((2, 1)))
