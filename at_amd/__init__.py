@@ -30,3 +30,4 @@ for _name in ("find_orbit", "find_orbit4", "find_orbit6", "find_sync_orbit", "fi
 for _name in ("get_acceptance", "get_1d_acceptance", "get_horizontal_acceptance",
               "get_vertical_acceptance", "get_momentum_acceptance"):
     setattr(Lattice, _name, getattr(_acceptance, _name))
+from .formats import load_json, load_lattice, load_repr, save_json, save_lattice, save_repr  # noqa: E402,F401
