@@ -27,7 +27,11 @@ enum Kind : int32_t {
   K_THIN = 6,     // ThinMPolePass
   K_RF = 7,       // RFCavityPass
   K_CORR = 8,     // CorrectorPass
-  K_APERTURE = 9  // AperturePass
+  K_APERTURE = 9, // AperturePass
+  K_CAVITY = 10,      // CavityPass
+  K_EXDRIFT = 11,     // ExactDriftPass
+  K_EXMPOLE = 12,     // ExactMultipolePass
+  K_EXMPOLERAD = 13   // ExactMultipoleRadPass
 };
 
 // Header flags.
@@ -71,6 +75,7 @@ enum Slot : int32_t {
   S_RADC = 16,
   S_BAX = 17, S_BAY = 18,
   S_RK1 = 19, S_RK2 = 20,                   // S_RADC * S_K1, S_RADC * S_K2 (fast build)
+  S_ELEN = 21,                              // Length (exact multipoles)
   S_NSCALAR = 22,
   // RF cavity
   S_RF_HALF = 0, S_RF_NV = 1, S_RF_TWOPIF = 2, S_RF_LAG = 3, S_RF_PHILAG = 4, S_RF_HFT0 = 5,

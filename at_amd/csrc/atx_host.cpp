@@ -67,6 +67,10 @@ const PassInfo kPasses[] = {
     {"ThinMPolePass", K_THIN, true},
     {"CorrectorPass", K_CORR, false},
     {"AperturePass", K_APERTURE, false},
+    {"CavityPass", K_CAVITY, false},
+    {"ExactDriftPass", K_EXDRIFT, false},
+    {"ExactMultipolePass", K_EXMPOLE, true},
+    {"ExactMultipoleRadPass", K_EXMPOLERAD, true},
 };
 
 struct HostElem {
@@ -317,9 +321,25 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       transforms(rd, h, pool);
       break;
     case K_DRIFT:
+    case K_EXDRIFT:  // DriftPass.c:50-80, ExactDriftPass.c:46-75
       sc[S_D_LEN] = rd.dbl("Length");
       transforms(rd, h, pool);
       break;
+    case K_CAVITY: {  // CavityPass.c:62-88
+      const double len = rd.dbl("Length");
+      const double volt = rd.dbl("Voltage");
+      const double freq = rd.dbl("Frequency");
+      const double elem_energy = rd.opt_dbl("Energy", energy);
+      const double lag = rd.opt_dbl("TimeLag", 0.0);
+      const double philag = rd.opt_dbl("PhaseLag", 0.0);
+      if (rd.err) break;
+      sc[S_RF_HALF] = len / 2;
+      sc[S_RF_NV] = volt / (energy == 0.0 ? elem_energy : energy);
+      sc[S_RF_TWOPIF] = kTwoPi * freq;
+      sc[S_RF_LAG] = lag;
+      sc[S_RF_PHILAG] = philag;
+      if (len != 0.0) h.flags |= F_THICK;
+    } break;
     case K_APERTURE: {
       auto lim = rd.arr("Limits");
       if (lim && lim->size() < 4) rd.bad("Limits must have 4 values");
@@ -365,7 +385,8 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
     default: {  // multipoles
       const bool thin = (e.kind == K_THIN);
       const bool bend = (e.kind == K_BND || e.kind == K_BNDRAD);
-      const bool rad = (e.kind == K_STRRAD || e.kind == K_BNDRAD);
+      const bool exact = (e.kind == K_EXMPOLE || e.kind == K_EXMPOLERAD);
+      const bool rad = (e.kind == K_STRRAD || e.kind == K_BNDRAD || e.kind == K_EXMPOLERAD);
       const double len = thin ? 0.0 : rd.dbl("Length");
       auto pa = rd.arr("PolynomA");
       auto pb = rd.arr("PolynomB");
@@ -380,6 +401,10 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       if (rd.err) break;
       if (order < 0 || order > 30) {
         rd.bad("MaxOrder out of range");
+        break;
+      }
+      if (exact && nslice <= 0) {  // ExactMultipolePass.c:126-128
+        rd.err = fail(ATX_E_VALUE, "NumIntSteps must be positive");
         break;
       }
       const size_t need = (size_t)order + 1 > (thin ? 1u : 2u) ? (size_t)order + 1 : (thin ? 1u : 2u);
@@ -421,6 +446,25 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       sc[S_L2] = sl * kDrift2;
       sc[S_K1] = sl * kKick1;
       sc[S_K2] = sl * kKick2;
+      sc[S_ELEN] = len;
+      if (exact) {
+        // PTC multipole fringes of every order whenever the flag is set
+        // (ExactMultipolePass.c:62-63,83-84); the Forest-Ruth scheme of the
+        // CPU pass whatever the integrator type
+        const long fqi = rd.opt_lng("FringeQuadEntrance", 0);
+        const long fqo = rd.opt_lng("FringeQuadExit", 0);
+        if (rd.err) break;
+        h.fringe = (fqi ? 1 : 0) | ((fqo ? 1 : 0) << 8);
+        if (rad) {
+          if (!(energy > 0.0)) {
+            rd.err = fail(ATX_E_VALUE, "Energy must be positive. Check lattice or set the track() keyword argument.");
+            break;
+          }
+          const double gamma = (rest == 0.0) ? 1.0E-9 * energy / kE0GeV : energy / rest;
+          sc[S_RADC] = kRadConst * std::pow(gamma, 3);
+        }
+        break;
+      }
       if (integrator != 4) {
         // op table: drifts and kicks of SL*c_i / SL*d_i, zero steps omitted
         // as atgpu's generated slice loop omits them (SymplecticIntegrator.cpp:140-171)
@@ -499,6 +543,8 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
   if ((h.flags & F_T1) && pool[h.t1 + 4] != 0.0) aerr = "T1 shifts delta";
   if ((h.flags & F_T2) && pool[h.t2 + 4] != 0.0) aerr = "T2 shifts delta";
   if (h.kind == K_STRRAD || h.kind == K_BNDRAD) aerr = "radiating pass";
+  if (h.kind == K_CAVITY || h.kind == K_EXDRIFT || h.kind == K_EXMPOLE || h.kind == K_EXMPOLERAD)
+    aerr = "pass method without an adjoint";
   if (h.flags & F_SCALE) aerr = "FieldScaling != 1";
   if (h.fringe) aerr = "quadrupole fringe";
   if (h.ops >= 0) aerr = "integrator other than 4";

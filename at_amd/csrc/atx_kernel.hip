@@ -618,6 +618,144 @@ __device__ __forceinline__ void rf_cavity(R6& r, cDevElem& h, cdouble* pool, int
   }
 }
 
+// CavityPass.c:18-56: no turn dependence (nv = V/E prepared on the host).
+__device__ __noinline__ R6 cavity(R6 r, cdouble* s, bool thick) {
+  if (thick) drift6(r, s[S_RF_HALF]);
+  r.dp += -s[S_RF_NV] * sin(s[S_RF_TWOPIF] * (r.ct - s[S_RF_LAG]) / 2.99792458e8 - s[S_RF_PHILAG]);
+  if (thick) drift6(r, s[S_RF_HALF]);
+  return r;
+}
+
+// Exact drift, absolute path length (exactdrift.c:4-16).
+__device__ __forceinline__ void exact_drift(R6& r, double L) {
+  const double opd = 1.0 + r.dp;
+  const double pz = sqrt(opd * opd - r.px * r.px - r.py * r.py);
+  const double nl = L / pz;
+  r.x += r.px * nl;
+  r.y += r.py * nl;
+  r.ct += nl * opd;
+}
+
+// The same drift in the form of the radiating pass (diff_exactdrift.c:30-43).
+__device__ __forceinline__ void exact_rad_drift(R6& r, double L) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double nl = L * pn / sqrt(1.0 - xpr * xpr - ypr * ypr);
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * (1.0 + r.dp);
+}
+
+// Radiating straight kick with the exact velocity (diff_str_exactkick.c:14-71).
+__device__ __forceinline__ void exact_rad_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
+                                               double radc) {
+  double re, im;
+  field<-1>(r, a, b, n, re, im);
+  double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double t = im * xpr + re * ypr;
+  const double b2p = im * im + re * re - t * t;
+  const double factor = kl / (pn * pn) / sqrt(1.0 - xpr * xpr - ypr * ypr);
+  r.dp -= radc * b2p * factor;
+  pn = 1.0 / (1.0 + r.dp);
+  r.px = xpr / pn;
+  r.py = ypr / pn;
+  r.px -= kl * re;
+  r.py += kl * im;
+}
+
+// PTC multipole fringe (Forest 13.29) of every order, edge = +1 at the
+// entrance, -1 at the exit (exactmultipolefringe.c:2-95, skip_b0 = 0).
+__device__ __noinline__ R6 multipole_fringe(R6 r, cdouble* pa, cdouble* pb, int order, double edge) {
+  double fx = 0, fy = 0, fx_x = 0, fx_y = 0, fy_x = 0, fy_y = 0;
+  double rx = 1.0, ix = 0.0;
+  for (int k = 0; k <= order; k++) {
+    const double bn = pb[k], an = pa[k];
+    const int j = k + 1;
+    const double drx = rx, dix = ix;
+    rx = drx * r.x - dix * r.y;
+    ix = drx * r.y + dix * r.x;
+    double u = bn * rx - an * ix;
+    double v = bn * ix + an * rx;
+    double du = bn * drx - an * dix;
+    double dv = bn * dix + an * drx;
+    const double f1 = -edge / 4.0 / (j + 1);
+    u = u * f1;
+    v = v * f1;
+    du = du * f1;
+    dv = dv * f1;
+    const double dux = j * du, dvx = j * dv, duy = -j * dv, dvy = j * du;
+    const double nf = 1.0 * (j + 2) / j;
+    fx += u * r.x + nf * v * r.y;
+    fy += u * r.y - nf * v * r.x;
+    fx_x += dux * r.x + u + nf * r.y * dvx;
+    fx_y += duy * r.x + nf * v + nf * r.y * dvy;
+    fy_x += dux * r.y - nf * v - nf * r.x * dvx;
+    fy_y += duy * r.y + u - nf * r.x * dvy;
+  }
+  const double del = 1.0 / (1 + r.dp);
+  const double ma = 1 - fx_x * del;
+  const double mb = -fy_x * del;
+  const double md = 1 - fy_y * del;
+  const double mc = -fx_y * del;
+  r.x = r.x - fx * del;
+  r.y = r.y - fy * del;
+  const double pxf = (md * r.px - mb * r.py) / (ma * md - mb * mc);
+  const double pyf = (ma * r.py - mc * r.px) / (ma * md - mb * mc);
+  r.py = pyf;
+  r.px = pxf;
+  r.ct = r.ct - (r.px * fx + r.py * fy) * del * del;
+  return r;
+}
+
+// ExactMultipolePass.c:27-96 / ExactMultipoleRadPass.c:26-104 slices, out
+// of line (generic order): these passes are rare in rings.
+__device__ __noinline__ R6 exact_slices(R6 r, cdouble* a, cdouble* b, int n, int nslice,
+                                        cdouble* s, bool rad) {
+  const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
+  if (rad) {
+    const double radc = s[S_RADC];
+    for (int m = 0; m < nslice; m++) {
+      exact_rad_drift(r, l1);
+      exact_rad_kick(r, a, b, n, k1, radc);
+      exact_rad_drift(r, l2);
+      exact_rad_kick(r, a, b, n, k2, radc);
+      exact_rad_drift(r, l2);
+      exact_rad_kick(r, a, b, n, k1, radc);
+      exact_rad_drift(r, l1);
+    }
+  } else {
+    for (int m = 0; m < nslice; m++) {
+      exact_drift(r, l1);
+      str_kick<-1>(r, a, b, n, k1);
+      exact_drift(r, l2);
+      str_kick<-1>(r, a, b, n, k2);
+      exact_drift(r, l2);
+      str_kick<-1>(r, a, b, n, k1);
+      exact_drift(r, l1);
+    }
+  }
+  return r;
+}
+
+__device__ __forceinline__ void exact_mpole(R6& r, cDevElem& h, cdouble* pool, bool rad) {
+  cdouble* s = pool + h.sc;
+  cdouble* a = pool + h.poly;
+  cdouble* b = a + h.order + 1;
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  if (h.fringe & 0xff) r = multipole_fringe(r, a, b, h.order, 1.0);
+  r = exact_slices(r, a, b, h.order, h.nslice, s, rad);
+  r.ct -= s[S_ELEN];  // absolute path length -> path lengthening
+  if (h.fringe & 0xff00) r = multipole_fringe(r, a, b, h.order, -1.0);
+  apertures(r, h, pool);
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
 // CorrectorPass.c:25-90
 __device__ __forceinline__ void corrector(R6& r, cDevElem& h, cdouble* pool) {
   cdouble* s = pool + h.sc;
@@ -666,6 +804,17 @@ __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int n
     case K_RF: rf_cavity(r, h, pool, nturn); break;
     case K_CORR: corrector(r, h, pool); break;
     case K_APERTURE: rect_ap(r, pool + h.rap); break;  // AperturePass.c:9-26
+    case K_CAVITY: r = cavity(r, pool + h.sc, (h.flags & F_THICK) != 0); break;
+    case K_EXDRIFT:  // ExactDriftPass.c:15-44
+      entry_xform(r, h, pool);
+      apertures(r, h, pool);
+      exact_drift(r, pool[h.sc + S_D_LEN]);
+      r.ct -= pool[h.sc + S_D_LEN];
+      apertures(r, h, pool);
+      exit_xform(r, h, pool);
+      break;
+    case K_EXMPOLE: exact_mpole(r, h, pool, false); break;
+    case K_EXMPOLERAD: exact_mpole(r, h, pool, true); break;
     default: break;
   }
 }

@@ -14,8 +14,10 @@ Per element with n = MaxOrder and s = NumIntSteps:
   BndMPoleSymplectic4Pass 14 + s(66 + 24n); StrMPoleSymplectic4RadPass
   2 + s(144 + 24n); BndMPoleSymplectic4RadPass 10 + s(177 + 24n);
   ThinMPolePass 8n + 2 (+8 with BendingAngle); RFCavityPass 32 (L > 0) or
-  4 (L = 0); CorrectorPass 23 (thick) or 2 (thin); Aperture/Identity 0;
-  +6 per T1/T2, +66 per R1/R2.
+  4 (L = 0); CavityPass 34 (L > 0) or 6 (L = 0); CorrectorPass 23 (thick)
+  or 2 (thin); ExactDriftPass 14; ExactMultipolePass 1 + s(64 + 24n)
+  (+ 54(n+1) + 30 per PTC fringe); ExactMultipoleRadPass 1 + s(161 + 24n)
+  (+ fringes); Aperture/Identity 0; +6 per T1/T2, +66 per R1/R2.
 These are the figures the benchmark divides by the kernel time to report
 achieved FP64 FLOP/s; they are a property of the algorithm, not of the
 kernel's instruction count.
@@ -47,9 +49,18 @@ def element_flops(e) -> int:
         f = 32 if L > 0 else 4
     elif pm == "CorrectorPass":
         f = 23 if L != 0 else 2
+    elif pm == "CavityPass":
+        f = 34 if L > 0 else 6
+    elif pm == "ExactDriftPass":
+        f = 14
+    elif pm in ("ExactMultipolePass", "ExactMultipoleRadPass"):
+        f = 1 + s * ((161 if pm.endswith("RadPass") else 64) + 24 * n)
+        nfringe = int(bool(getattr(e, "FringeQuadEntrance", 0))) + int(
+            bool(getattr(e, "FringeQuadExit", 0)))
+        f += nfringe * (54 * (n + 1) + 30)
     else:  # AperturePass, IdentityPass
         f = 0
-    if pm not in ("AperturePass", "RFCavityPass"):
+    if pm not in ("AperturePass", "RFCavityPass", "CavityPass"):
         f += 6 * (getattr(e, "T1", None) is not None) + 6 * (getattr(e, "T2", None) is not None)
         f += 66 * (getattr(e, "R1", None) is not None) + 66 * (getattr(e, "R2", None) is not None)
     return f

@@ -239,6 +239,94 @@ static void or_rad_bnd_kick(double *r, const double *a, const double *b, int n,
 /* Edge focusing: atphyslib.c:15-79 (non-radiating) and                   */
 /* diff_bend_fringe.c:2-53 (radiating, B propagation omitted)              */
 
+/* exactdrift.c:4-16 (ExactDriftPass, ExactMultipolePass): absolute path */
+static void or_exact_drift(double *r, double L)
+{
+    double pz = sqrt((1 + r[4]) * (1 + r[4]) - r[1] * r[1] - r[3] * r[3]);
+    double nl = L / pz;
+    r[0] += r[1] * nl;
+    r[2] += r[3] * nl;
+    r[5] += nl * (1.0 + r[4]);
+}
+
+/* diff_exactdrift.c:30-43 (ExactMultipoleRadPass), bdiff == NULL */
+static void or_exact_rad_drift(double *r, double L)
+{
+    double pn = 1.0 / (1.0 + r[4]);
+    double xpr = r[1] * pn;
+    double ypr = r[3] * pn;
+    double nl = L * pn / sqrt(1.0 - xpr * xpr - ypr * ypr);
+    r[0] += nl * r[1];
+    r[2] += nl * r[3];
+    r[5] += nl * (1.0 + r[4]);
+}
+
+/* diff_str_exactkick.c:14-71, bdiff == NULL */
+static void or_exact_rad_kick(double *r, const double *a, const double *b, int n, double L,
+                              double rad_const)
+{
+    double re, im;
+    or_field(r, a, b, n, &re, &im);
+    double pn = 1.0 / (1.0 + r[4]);
+    double xpr = r[1] * pn;
+    double ypr = r[3] * pn;
+    /* B2perp(bx = ImSum, by = ReSum): bx^2 + by^2 - (bx xpr + by ypr)^2 */
+    double t = im * xpr + re * ypr;
+    double b2p = im * im + re * re - t * t;
+    double factor = L / (pn * pn) / sqrt(1.0 - xpr * xpr - ypr * ypr);
+    r[4] -= rad_const * b2p * factor;
+    pn = 1.0 / (1.0 + r[4]);
+    r[1] = xpr / pn;
+    r[3] = ypr / pn;
+    r[1] -= L * re;
+    r[3] += L * im;
+}
+
+/* exactmultipolefringe.c:2-95 (PTC multipole fringe, Forest 13.29),
+ * skip_b0 = 0; edge = +1 at the entrance, -1 at the exit */
+static void or_multipole_fringe(double *r, const double *pa, const double *pb, int max_order,
+                                double edge)
+{
+    double fx = 0, fy = 0, fx_x = 0, fx_y = 0, fy_x = 0, fy_y = 0;
+    double rx = 1.0, ix = 0.0;
+    for (int n = 0; n <= max_order; n++) {
+        double bn = pb[n], an = pa[n];
+        int j = n + 1;
+        double drx = rx, dix = ix;
+        rx = drx * r[0] - dix * r[2];
+        ix = drx * r[2] + dix * r[0];
+        double u = bn * rx - an * ix;
+        double v = bn * ix + an * rx;
+        double du = bn * drx - an * dix;
+        double dv = bn * dix + an * drx;
+        double f1 = -edge / 4.0 / (j + 1);
+        u = u * f1;
+        v = v * f1;
+        du = du * f1;
+        dv = dv * f1;
+        double dux = j * du, dvx = j * dv, duy = -j * dv, dvy = j * du;
+        double nf = 1.0 * (j + 2) / j;
+        fx += u * r[0] + nf * v * r[2];
+        fy += u * r[2] - nf * v * r[0];
+        fx_x += dux * r[0] + u + nf * r[2] * dvx;
+        fx_y += duy * r[0] + nf * v + nf * r[2] * dvy;
+        fy_x += dux * r[2] - nf * v - nf * r[0] * dvx;
+        fy_y += duy * r[2] + u - nf * r[0] * dvy;
+    }
+    double del = 1.0 / (1 + r[4]);
+    double ma = 1 - fx_x * del;
+    double mb = -fy_x * del;
+    double md = 1 - fy_y * del;
+    double mc = -fx_y * del;
+    r[0] = r[0] - fx * del;
+    r[2] = r[2] - fy * del;
+    double pxf = (md * r[1] - mb * r[3]) / (ma * md - mb * mc);
+    double pyf = (ma * r[3] - mc * r[1]) / (ma * md - mb * mc);
+    r[3] = pyf;
+    r[1] = pxf;
+    r[5] = r[5] - (r[1] * fx + r[3] * fy) * del * del;
+}
+
 static double or_fringe_corr(double irho, double edge, double fint, double gap, int method)
 {
     if ((fint == 0.0) || (gap == 0.0) || (method == 0)) return 0.0;
@@ -412,8 +500,13 @@ static int or_prepare(const oracle_elem *e, or_prep *p, double energy, double re
     case OR_STRMPOLESYMPLECTIC4RADPASS:
     case OR_BNDMPOLESYMPLECTIC4PASS:
     case OR_BNDMPOLESYMPLECTIC4RADPASS:
+    case OR_EXACTMULTIPOLEPASS:
+    case OR_EXACTMULTIPOLERADPASS:
     case OR_THINMPOLEPASS: {
         int n = e->MaxOrder;
+        if ((e->pass == OR_EXACTMULTIPOLEPASS || e->pass == OR_EXACTMULTIPOLERADPASS) &&
+            e->NumIntSteps <= 0)
+            return or_fail(3, "NumIntSteps must be positive"); /* ExactMultipolePass.c:126-128 */
         if (n < 0 || n > 30) return or_fail(2, "MaxOrder out of range");
         if (!e->PolynomA || !e->PolynomB) return or_fail(2, "PolynomA/PolynomB missing");
         p->order = n;
@@ -445,7 +538,8 @@ static int or_prepare(const oracle_elem *e, or_prep *p, double energy, double re
         }
         if (e->pass == OR_BNDMPOLESYMPLECTIC4PASS || e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS)
             p->irho = e->BendingAngle / e->Length;
-        if (e->pass == OR_STRMPOLESYMPLECTIC4RADPASS || e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS) {
+        if (e->pass == OR_STRMPOLESYMPLECTIC4RADPASS || e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS ||
+            e->pass == OR_EXACTMULTIPOLERADPASS) {
             /* atelem.c:203-211,364-371 */
             if (!(energy > 0.0))
                 return or_fail(3, "Energy must be positive. Check lattice or set the track() keyword argument.");
@@ -460,6 +554,9 @@ static int or_prepare(const oracle_elem *e, or_prep *p, double energy, double re
         p->harm = round(e->Frequency * T0); /* RFCavityPass.c:59 */
         p->nv = e->Voltage / energy;        /* RFCavityPass.c:65 */
         break;
+    case OR_CAVITYPASS: /* CavityPass.c:66-88: element Energy when the ring's is 0 */
+        p->nv = e->Voltage / (energy == 0.0 ? e->Energy : energy);
+        break;
     case OR_CORRECTORPASS:
         if (!e->KickAngle) return or_fail(2, "The required attribute KickAngle is missing.");
         break;
@@ -468,6 +565,7 @@ static int or_prepare(const oracle_elem *e, or_prep *p, double energy, double re
         break;
     case OR_IDENTITYPASS:
     case OR_DRIFTPASS:
+    case OR_EXACTDRIFTPASS:
         break;
     default:
         return or_fail(4, "PassMethod not in the oracle");
@@ -598,6 +696,61 @@ static void or_pass(double *r, const oracle_elem *e, const or_prep *p, int nturn
             or_drift6(r, half);
             if (p->nv != 0.0)
                 r[4] += -p->nv * sin(OR_TWOPI * freq * ((r[5] - e->TimeLag) / OR_C0 - (p->harm / freq - T0) * nturn) - e->PhaseLag);
+            or_drift6(r, half);
+        }
+        break;
+    }
+    case OR_EXACTDRIFTPASS: /* ExactDriftPass.c:15-44 */
+        or_entry_transform(r, e);
+        or_apertures(r, e);
+        or_exact_drift(r, e->Length);
+        r[5] -= e->Length;
+        or_apertures(r, e);
+        or_exit_transform(r, e);
+        break;
+    case OR_EXACTMULTIPOLEPASS:    /* ExactMultipolePass.c:27-96 */
+    case OR_EXACTMULTIPOLERADPASS: /* ExactMultipoleRadPass.c:26-104 */
+    {
+        const int rad = (e->pass == OR_EXACTMULTIPOLERADPASS);
+        if (s != 1.0) or_change_pref(r, s);
+        or_entry_transform(r, e);
+        or_apertures(r, e);
+        if (e->FringeQuadEntrance) or_multipole_fringe(r, p->a, p->b, p->order, 1.0);
+        for (int m = 0; m < e->NumIntSteps; m++) {
+            if (rad) {
+                or_exact_rad_drift(r, p->l1);
+                or_exact_rad_kick(r, p->a, p->b, p->order, p->k1, p->rad_const);
+                or_exact_rad_drift(r, p->l2);
+                or_exact_rad_kick(r, p->a, p->b, p->order, p->k2, p->rad_const);
+                or_exact_rad_drift(r, p->l2);
+                or_exact_rad_kick(r, p->a, p->b, p->order, p->k1, p->rad_const);
+                or_exact_rad_drift(r, p->l1);
+            } else {
+                or_exact_drift(r, p->l1);
+                or_str_kick(r, p->a, p->b, p->k1, p->order);
+                or_exact_drift(r, p->l2);
+                or_str_kick(r, p->a, p->b, p->k2, p->order);
+                or_exact_drift(r, p->l2);
+                or_str_kick(r, p->a, p->b, p->k1, p->order);
+                or_exact_drift(r, p->l1);
+            }
+        }
+        r[5] -= e->Length;
+        if (e->FringeQuadExit) or_multipole_fringe(r, p->a, p->b, p->order, -1.0);
+        or_apertures(r, e);
+        or_exit_transform(r, e);
+        if (s != 1.0) or_change_pref(r, p->inv_scaling);
+        break;
+    }
+    case OR_CAVITYPASS: /* CavityPass.c:18-56 */
+    {
+        const double le = e->Length;
+        if (le == 0) {
+            r[4] += -p->nv * sin(OR_TWOPI * e->Frequency * (r[5] - e->TimeLag) / OR_C0 - e->PhaseLag);
+        } else {
+            double half = le / 2;
+            or_drift6(r, half);
+            r[4] += -p->nv * sin(OR_TWOPI * e->Frequency * (r[5] - e->TimeLag) / OR_C0 - e->PhaseLag);
             or_drift6(r, half);
         }
         break;

@@ -37,6 +37,9 @@ enum {
     OR_STRMPOLESYMPLECTIC4RADPASS = 5,
     OR_CAVITYPASS = 6,
     OR_RFCAVITYPASS = 7,
+    OR_EXACTDRIFTPASS = 8,
+    OR_EXACTMULTIPOLEPASS = 9,
+    OR_EXACTMULTIPOLERADPASS = 10,
     OR_THINMPOLEPASS = 11,
     OR_CORRECTORPASS = 12,
     OR_APERTUREPASS = 13
@@ -65,6 +68,7 @@ typedef struct oracle_elem {
     double PhaseLag;
     double bax;            /* ThinMPolePass BendingAngle[0] */
     double bay;            /* ThinMPolePass BendingAngle[1] */
+    double Energy;         /* element Energy (CavityPass), 0 when absent */
     const double *PolynomA;
     const double *PolynomB;
     const double *fringeIntM0;

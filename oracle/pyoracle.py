@@ -28,7 +28,11 @@ PASS_IDS = {
     "BndMPoleSymplectic4RadPass": 3,
     "StrMPoleSymplectic4Pass": 4,
     "StrMPoleSymplectic4RadPass": 5,
+    "CavityPass": 6,
     "RFCavityPass": 7,
+    "ExactDriftPass": 8,
+    "ExactMultipolePass": 9,
+    "ExactMultipoleRadPass": 10,
     "ThinMPolePass": 11,
     "CorrectorPass": 12,
     "AperturePass": 13,
@@ -46,7 +50,7 @@ class OracleElem(C.Structure):
         ("ExitAngle", C.c_double), ("FringeInt1", C.c_double), ("FringeInt2", C.c_double),
         ("FullGap", C.c_double), ("FieldScaling", C.c_double), ("Voltage", C.c_double),
         ("Frequency", C.c_double), ("TimeLag", C.c_double), ("PhaseLag", C.c_double),
-        ("bax", C.c_double), ("bay", C.c_double),
+        ("bax", C.c_double), ("bay", C.c_double), ("Energy", C.c_double),
         ("PolynomA", _dp), ("PolynomB", _dp), ("fringeIntM0", _dp), ("fringeIntP0", _dp),
         ("R1", _dp), ("R2", _dp), ("T1", _dp), ("T2", _dp), ("RApertures", _dp),
         ("EApertures", _dp), ("KickAngle", _dp), ("Limits", _dp),
@@ -134,6 +138,7 @@ def make_elem(e, keep: list) -> OracleElem:
     s.Frequency = float(g("Frequency"))
     s.TimeLag = float(g("TimeLag"))
     s.PhaseLag = float(g("PhaseLag"))
+    s.Energy = float(g("Energy", 0.0))
     for name in _ARRAYS:
         v = getattr(e, name, None)
         if v is None:

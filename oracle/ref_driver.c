@@ -54,6 +54,15 @@ typedef void (*fn_rf)(double *, double, double, double, double, double, double, 
 typedef void (*fn_corr)(double *, double, double, double, double *, double *, double *,
                         double *, double *, double *, double, int);
 typedef void (*fn_ap)(double *, double *, int);
+typedef void (*fn_cav)(double *, double, double, double, double, double, int);
+typedef void (*fn_exdrift)(double *, double, const double *, const double *, const double *,
+                           const double *, double *, double *, int);
+typedef void (*fn_exmpole)(double *, double, double *, double *, int, int, int, int, double *,
+                           double *, double *, double *, double *, double *, double *, double,
+                           int);
+typedef void (*fn_exmpolerad)(double *, double, double *, double *, int, int, int, int,
+                              double *, double *, double *, double *, double *, double *,
+                              double *, double, double, int, double *);
 
 static char rf_err[1024];
 static char rf_dir[400] = "";
@@ -87,6 +96,10 @@ static const char *rf_name(int pass)
     case OR_THINMPOLEPASS: return "ThinMPolePass";
     case OR_CORRECTORPASS: return "CorrectorPass";
     case OR_APERTUREPASS: return "AperturePass";
+    case OR_CAVITYPASS: return "CavityPass";
+    case OR_EXACTDRIFTPASS: return "ExactDriftPass";
+    case OR_EXACTMULTIPOLEPASS: return "ExactMultipolePass";
+    case OR_EXACTMULTIPOLERADPASS: return "ExactMultipoleRadPass";
     default: return NULL;
     }
 }
@@ -135,10 +148,16 @@ static int rf_prepare(const oracle_elem *e, rf_prep *p, double energy, double re
     }
     if (e->pass == OR_BNDMPOLESYMPLECTIC4PASS || e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS)
         p->irho = e->BendingAngle / e->Length;
-    if (e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS || e->pass == OR_STRMPOLESYMPLECTIC4RADPASS) {
+    if (e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS || e->pass == OR_STRMPOLESYMPLECTIC4RADPASS ||
+        e->pass == OR_EXACTMULTIPOLERADPASS) {
         if (!(energy > 0.0)) return rf_fail(3, "Energy must be positive");
         p->gamma = (rest_energy == 0.0) ? 1.0E-9 * energy / RF_E0_GEV : energy / rest_energy;
     }
+    if (e->pass == OR_CAVITYPASS) /* CavityPass.c:84-86 */
+        p->nv = e->Voltage / (energy == 0.0 ? e->Energy : energy);
+    if ((e->pass == OR_EXACTMULTIPOLEPASS || e->pass == OR_EXACTMULTIPOLERADPASS) &&
+        e->NumIntSteps <= 0)
+        return rf_fail(3, "NumIntSteps must be positive");
     if (e->pass == OR_RFCAVITYPASS) {
         if (!(energy > 0.0)) return rf_fail(3, "Energy must be positive");
         p->harm = round(e->Frequency * T0);
@@ -209,6 +228,25 @@ static int rf_call(const oracle_elem *e, rf_prep *p, double *r, int np, int ntur
     case OR_APERTUREPASS:
         if (!e->Limits) return rf_fail(2, "Limits missing");
         ((fn_ap)f)(r, D(e->Limits), np);
+        break;
+    case OR_CAVITYPASS:
+        ((fn_cav)f)(r, e->Length, p->nv, e->Frequency, e->TimeLag, e->PhaseLag, np);
+        break;
+    case OR_EXACTDRIFTPASS:
+        ((fn_exdrift)f)(r, e->Length, e->T1, e->T2, e->R1, e->R2, D(e->RApertures),
+                        D(e->EApertures), np);
+        break;
+    case OR_EXACTMULTIPOLEPASS:
+        ((fn_exmpole)f)(r, e->Length, p->a, p->b, e->MaxOrder, e->NumIntSteps,
+                        e->FringeQuadEntrance, e->FringeQuadExit, D(e->T1), D(e->T2), D(e->R1),
+                        D(e->R2), D(e->RApertures), D(e->EApertures), D(e->KickAngle),
+                        e->FieldScaling, np);
+        break;
+    case OR_EXACTMULTIPOLERADPASS:
+        ((fn_exmpolerad)f)(r, e->Length, p->a, p->b, e->MaxOrder, e->NumIntSteps,
+                           e->FringeQuadEntrance, e->FringeQuadExit, D(e->T1), D(e->T2),
+                           D(e->R1), D(e->R2), D(e->RApertures), D(e->EApertures),
+                           D(e->KickAngle), e->FieldScaling, p->gamma, np, NULL);
         break;
     default:
         return rf_fail(4, "PassMethod not available");

@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TRANSCENDENTAL = {"rf_thin", "rf_thick_lag", "bend_fringe_m1", "bend_fringe_m2",
                   "bend_fringe_m3", "bend_rad_fringe_m1", "bend_rad_fringe_m2",
-                  "bend_rad_fringe_m3"}
+                  "bend_rad_fringe_m3", "cavity_thin", "cavity_thick_phase"}
 CASES = {c[0]: c for c in zoo.cases()}
 REFS3 = np.array([0, 1, 2], np.uint32)
 

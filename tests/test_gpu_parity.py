@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 TRANSCENDENTAL = {"rf_thin", "rf_thick_lag", "bend_fringe_m1", "bend_fringe_m2",
                   "bend_fringe_m3", "bend_rad_fringe_m1", "bend_rad_fringe_m2",
-                  "bend_rad_fringe_m3"}
+                  "bend_rad_fringe_m3", "cavity_thin", "cavity_thick_phase"}
 CASES = zoo.cases()
 IDS = [c[0] for c in CASES]
 

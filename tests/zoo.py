@@ -124,6 +124,27 @@ def cases() -> list:
     add("corrector_thin", Corrector("C", 0.0, np.array([1e-4, -2e-4])))
     add("corrector_thick", Corrector("C", 0.2, np.array([1e-4, -2e-4]), FieldScaling=1.001))
     add("corrector_thick_ap_tr", _ap(_misalign(Corrector("C", 0.1, np.array([3e-5, 1e-5])))))
+    # --- CavityPass
+    add("cavity_thin", RFCavity("C", 0.0, 2.0e6, 3.52e8, 992, 6.0e9, TimeLag=0.02,
+                                PassMethod="CavityPass"), E6)
+    add("cavity_thick_phase", RFCavity("C", 0.6, 4.0e6, 3.52e8, 992, 6.0e9, PhaseLag=0.2,
+                                       PassMethod="CavityPass"), E6)
+    # --- ExactDriftPass / ExactMultipolePass / ExactMultipoleRadPass
+    add("exact_drift", Drift("D", 1.3, PassMethod="ExactDriftPass"))
+    add("exact_drift_ap_tr", _ap(_misalign(Drift("D", 0.7, PassMethod="ExactDriftPass"))))
+    add("exact_quad", Quadrupole("Q", 0.4, 1.2, PassMethod="ExactMultipolePass"))
+    add("exact_quad_fringe", Quadrupole("Q", 0.5, -2.2, NumIntSteps=7, FringeQuadEntrance=1,
+                                        FringeQuadExit=1, PassMethod="ExactMultipolePass"))
+    add("exact_sext_kick_scaled", Sextupole("S", 0.2, 40.0, KickAngle=np.array([1e-4, -2e-4]),
+                                            FieldScaling=1.01, PassMethod="ExactMultipolePass"))
+    add("exact_mpole_skew_fringe", Multipole("M", 0.3, [0.0, 0.3, 4.0], [0.1, 0.9, -3.0, 50.0],
+                                             FringeQuadEntrance=1,
+                                             PassMethod="ExactMultipolePass"))
+    add("exact_quad_misaligned_ap", _ap(_misalign(Quadrupole("Q", 0.4, 1.7,
+                                                             PassMethod="ExactMultipolePass"))))
+    add("exact_quad_rad", Quadrupole("Q", 0.4, 3.0, PassMethod="ExactMultipoleRadPass"), E6)
+    add("exact_sext_rad_fringe", Sextupole("S", 0.25, 120.0, FringeQuadExit=1,
+                                           PassMethod="ExactMultipoleRadPass"), E6)
     return c
 
 
