@@ -36,6 +36,8 @@ enum Kind : int32_t {
 
 // Header flags.
 enum : uint32_t {
+  F_EDGEB_IN = 1u << 20,   // FringeInt1 > 0 and FullGap > 0 (diff_bend_fringe.c:36-44)
+  F_EDGEB_OUT = 1u << 21,  // FringeInt2 > 0 and FullGap > 0
   F_T1 = 1u << 0,
   F_R1 = 1u << 1,
   F_R2 = 1u << 2,
@@ -76,7 +78,8 @@ enum Slot : int32_t {
   S_BAX = 17, S_BAY = 18,
   S_RK1 = 19, S_RK2 = 20,                   // S_RADC * S_K1, S_RADC * S_K2 (fast build)
   S_ELEN = 21,                              // Length (exact multipoles)
-  S_NSCALAR = 22,
+  S_DIFC = 22,                              // DIF_CONST * gamma^5 (diffusion matrices)
+  S_NSCALAR = 24,
   // RF cavity
   S_RF_HALF = 0, S_RF_NV = 1, S_RF_TWOPIF = 2, S_RF_LAG = 3, S_RF_PHILAG = 4, S_RF_HFT0 = 5,
   // corrector
@@ -142,6 +145,17 @@ struct AdjElem {
   int32_t coeff;
   int32_t r1inv;
   int32_t r2inv;
+};
+
+// Diffusion matrices (one lane per element): B of each radiating element
+// from its entrance orbit.
+struct DiffArgs {
+  const DevElem* elems;
+  const double* pool;
+  const double* orbits;  // (6, nelem): orbit at the entrance of each element
+  double* bdiff;         // nelem x 36, column-major 6x6 per element
+  int32_t nelem;
+  int32_t pad;
 };
 
 struct AdjArgs {

@@ -16,8 +16,9 @@
 #include "../../include/atx.h"
 #include "atx_device.h"
 
-extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, hipStream_t);
-extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
 extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, hipStream_t);
 
 namespace atx {
@@ -42,6 +43,8 @@ const double kTwoPi = 2.0 * 3.14159265358979323846;
 const double kRe = 2.8179403205e-15;
 const double kE0GeV = 510.99895069e-6;
 const double kRadConst = 2.0 / 3.0 * kRe;
+// atconstants.h:24-31
+const double kDifConst = 55.0 / 24.0 / 1.7320508075688772 * 197.3269804e-18 / kE0GeV * kRe;
 
 struct Attr {
   std::vector<double> v;
@@ -169,6 +172,7 @@ struct atx_lattice {
   std::vector<atx::HostElem> elems;
   // device copy of the compiled lattice, valid for (energy, rest, elempass)
   bool dev_valid = false;
+  bool has_ext = false;  // CavityPass or exact passes present: extended kernel variant
   double dev_energy = -1.0, dev_rest = -1.0;
   bool dev_elempass = false;
   int dev_device = -1;
@@ -462,6 +466,7 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
           }
           const double gamma = (rest == 0.0) ? 1.0E-9 * energy / kE0GeV : energy / rest;
           sc[S_RADC] = kRadConst * std::pow(gamma, 3);
+          sc[S_DIFC] = kDifConst * std::pow(gamma, 5);
         }
         break;
       }
@@ -509,6 +514,8 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
         const int32_t eout_mode = edge_mode(rad, mo, irho, eout, f2, gap, &sc[S_FX_OUT], &sc[S_FY_OUT], &sc[S_CORR_OUT]);
         sc[S_EDGE_OUT] = eout;
         h.edge = ein_mode | (eout_mode << 8);
+        if (f1 > 0 && gap > 0) h.flags |= F_EDGEB_IN;
+        if (f2 > 0 && gap > 0) h.flags |= F_EDGEB_OUT;
       }
       if (rad) {
         if (!(energy > 0.0)) {
@@ -517,6 +524,7 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
         }
         const double gamma = (rest == 0.0) ? 1.0E-9 * energy / kE0GeV : energy / rest;
         sc[S_RADC] = kRadConst * std::pow(gamma, 3);
+        sc[S_DIFC] = kDifConst * std::pow(gamma, 5);
         sc[S_RK1] = sc[S_RADC] * sc[S_K1];
         sc[S_RK2] = sc[S_RADC] * sc[S_K2];
       }
@@ -628,9 +636,11 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
   lat->h_adjerr = c.adjerr;
   lat->h_order.clear();
   lat->h_kind.clear();
+  lat->has_ext = false;
   for (const auto& h : c.elems) {
     lat->h_order.push_back(h.order);
     lat->h_kind.push_back(h.kind);
+    lat->has_ext |= h.kind >= K_CAVITY;
   }
   lat->dev_valid = true;
   lat->dev_energy = energy;
@@ -648,7 +658,7 @@ constexpr int kBlock = 256;
 // slots[i] is the refpt slot table of instance i; instances that share a
 // table (same pointer) share its device copy.
 int launch_instances(std::vector<Instance>& insts, const std::vector<const std::vector<int32_t>*>& slots,
-                     int block, hipStream_t stream, bool strict) {
+                     int block, hipStream_t stream, bool strict, bool ext) {
   if (block <= 0) block = kBlock;
   if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
   int nblocks = 0;
@@ -681,8 +691,10 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   hipError_t ce = hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream);
   hipError_t le = hipSuccess;
   if (ce == hipSuccess)
-    le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block, stream)
-                : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block, stream);
+    le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
+                                          ext ? 1 : 0, stream)
+                : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
+                                        ext ? 1 : 0, stream);
   hipError_t fe = hipFreeAsync(d_scratch, stream);
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
@@ -729,7 +741,7 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   inst.block_begin = 0;
   inst.losses = losses ? 1 : 0;
   inst.elempass = elempass ? 1 : 0;
-  return launch_instances(insts, {&slots[0]}, kBlock, stream, strict);
+  return launch_instances(insts, {&slots[0]}, kBlock, stream, strict, lat->has_ext);
 }
 
 int check_partial(int nelem, int nturns, int& start, int& end) {
@@ -957,6 +969,7 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     if (items[i].nturns >= 0 && items[i].turn < 0)
       return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": turn must be >= 0");
   std::vector<Instance> insts;
+  bool ext = false;
   std::map<int, std::vector<int32_t>> empty_slots;  // refpts = [] table per ring length
   std::vector<const std::vector<int32_t>*> slots;
   std::vector<std::unique_lock<std::mutex>> locks;
@@ -972,6 +985,7 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     int rc = ensure_device_lattice(it.lattice, energy, rest_energy, false);
     if (rc) return fail(rc, "item " + std::to_string(i) + ": " + g_err);
     if (it.num_particles == 0) continue;
+    ext |= it.lattice->has_ext;
     const int nelem = (int)it.lattice->elems.size();
     Instance in;
     std::memset(&in, 0, sizeof(in));
@@ -1012,7 +1026,7 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     sorted_insts.push_back(insts[i]);
     sorted_slots.push_back(slots[i]);
   }
-  return launch_instances(sorted_insts, sorted_slots, block_size, stream, strict);
+  return launch_instances(sorted_insts, sorted_slots, block_size, stream, strict, ext);
 }
 
 int atx_element_set_array_at(atx_lattice* lat, int32_t index, const char* name, const double* data,
@@ -1083,6 +1097,56 @@ int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t 
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("adjoint launch: ") + hipGetErrorString(le));
   if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  return ATX_OK;
+}
+
+int atx_diffusion_matrices(atx_lattice* lat, const double* orbits, double* bdiff, double energy,
+                           double rest_energy, void* stream_, uint32_t flags) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  std::lock_guard<std::mutex> g(lat->mu);
+  const int nelem = (int)lat->elems.size();
+  if (nelem == 0) return ATX_OK;
+  if (!orbits || !bdiff) return fail(ATX_E_VALUE, "orbits/bdiff are NULL");
+  int rc = ensure_device_lattice(lat, energy, rest_energy, false);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool dev = (flags & ATX_DEVICE_POINTERS) != 0;
+  const size_t no = 6 * (size_t)nelem, nb = 36 * (size_t)nelem;
+  double* d_o = nullptr;
+  double* d_b = nullptr;
+  if (dev) {
+    d_o = const_cast<double*>(orbits);
+    d_b = bdiff;
+  } else {
+    HIPCHK(hipMallocAsync((void**)&d_o, (no + nb) * sizeof(double), stream));
+    d_b = d_o + no;
+    hipError_t ce = hipMemcpyAsync(d_o, orbits, no * sizeof(double), hipMemcpyHostToDevice, stream);
+    if (ce != hipSuccess) {
+      (void)hipFreeAsync(d_o, stream);
+      return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+    }
+  }
+  DiffArgs a;
+  a.elems = lat->d_elems;
+  a.pool = lat->d_pool;
+  a.orbits = d_o;
+  a.bdiff = d_b;
+  a.nelem = nelem;
+  a.pad = 0;
+  hipError_t le = atx_launch_diffusion(&a, stream);
+  if (dev) {
+    if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("diffusion launch: ") + hipGetErrorString(le));
+    return ATX_OK;
+  }
+  hipError_t ce = hipSuccess;
+  if (le == hipSuccess) ce = hipMemcpyAsync(bdiff, d_b, nb * sizeof(double), hipMemcpyDeviceToHost, stream);
+  hipError_t fe = hipFreeAsync(d_o, stream);
+  hipError_t se = hipStreamSynchronize(stream);
+  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("diffusion launch: ") + hipGetErrorString(le));
+  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  if (se != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
   return ATX_OK;
 }
 

@@ -782,7 +782,334 @@ __device__ __forceinline__ void corrector(R6& r, cDevElem& h, cdouble* pool) {
   if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
 }
 
+#if ATX_STRICT  // diffusion matrices: built with the reference operation order only
+// --- Diffusion matrices -----------------------------------------------------
+// Restatement of the reference's Ohmi diffusion-matrix propagation
+// (diff_drift.c, diff_thinkick.c, diff_str_kick.c, diff_bnd_kick.c,
+// diff_bend_fringe.c, diff_exactdrift.c, diff_str_exactkick.c and
+// ATsandwichmmt/ATaddmm of atlalib.c:92-158), used with bdiff != NULL by
+// the radiating passes. Matrices are column-major, [i][j] at [i + 6 j], and
+// the products keep the reference summation order.
+
+// B <- M B M^T (atlalib.c:129-147)
+__device__ __forceinline__ void sandwich(const double* m, double* b) {
+  double t[36];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double c = 0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) c += m[j + 6 * k] * b[k + 6 * i];
+      t[j + 6 * i] = c;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double c = 0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) c += t[i + 6 * k] * m[j + 6 * k];
+      b[i + 6 * j] = c;
+    }
+  }
+}
+
+__device__ __forceinline__ void eye6(double* m) {
+#pragma unroll
+  for (int i = 0; i < 36; i++) m[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) m[7 * i] = 1.0;
+}
+
+// diff_drift.c:2-43
+__device__ __forceinline__ void d_drift(R6& r, double L, double* b) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double nl = L * pn;
+  double m[36];
+  eye6(m);
+  m[6] = nl;
+  m[20] = nl;
+  m[24] = -nl * xpr;
+  m[26] = -nl * ypr;
+  m[11] = nl * xpr;
+  m[23] = nl * ypr;
+  m[29] = -nl * (xpr * xpr + ypr * ypr);
+  sandwich(m, b);
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * pn * (r.px * r.px + r.py * r.py) / 2.0;
+}
+
+// diff_exactdrift.c:2-43
+__device__ __forceinline__ void d_exact_drift(R6& r, double L, double* b) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double nl = L * pn / sqrt(1.0 - xpr * xpr - ypr * ypr);
+  const double den = 1.0 - xpr * xpr - ypr * ypr;
+  double m[36];
+  eye6(m);
+  m[6] = nl * (1.0 - ypr * ypr) / den;
+  m[18] = nl * xpr * ypr / den;
+  m[24] = -nl * xpr / den;
+  m[8] = nl * xpr * ypr / den;
+  m[20] = nl * (1.0 - xpr * xpr) / den;
+  m[26] = -nl * ypr / den;
+  m[11] = nl * xpr / den;
+  m[23] = nl * ypr / den;
+  m[29] = -nl * (xpr * xpr + ypr * ypr) / den;
+  sandwich(m, b);
+  r.x += nl * r.px;
+  r.y += nl * r.py;
+  r.ct += nl * (1.0 + r.dp);
+}
+
+// thinkickM of diff_thinkick.c:1-35 (irho = 0) and diff_bnd_kick.c:1-46
+__device__ __forceinline__ void d_kick_m(const R6& r, cdouble* a, cdouble* bp, int n, double L,
+                                         double irho, bool bend, double* b) {
+  double imn = n * a[n];
+  double ren = n * bp[n];
+  for (int k = n - 1; k > 0; k--) {
+    const double t = (ren * r.x - imn * r.y) + k * bp[k];
+    imn = imn * r.x + ren * r.y + k * a[k];
+    ren = t;
+  }
+  double m[36];
+  eye6(m);
+  m[1] = -L * ren;
+  m[13] = L * imn;
+  m[3] = L * imn;
+  m[15] = L * ren;
+  if (bend) {
+    m[25] = L * irho;
+    m[1] += -L * irho * irho;
+    m[5] = L * irho;
+  }
+  sandwich(m, b);
+}
+
+// thinkickB (diff_thinkick.c:37-71, diff_bnd_kick.c:48-83)
+__device__ __forceinline__ void d_kick_b(const R6& r, double difc, double b2p, double factor,
+                                         double* b) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  const double pn2 = pn * pn;
+  const double b3p = b2p * sqrt(b2p);
+  const double bb = difc * b3p * factor / pn2;
+  double a[36];
+#pragma unroll
+  for (int i = 0; i < 36; i++) a[i] = 0.0;
+  a[7] = bb * (r.px * r.px) * pn2;
+  a[19] = bb * r.px * r.py * pn2;
+  a[9] = a[19];
+  a[21] = bb * (r.py * r.py) * pn2;
+  a[10] = bb * r.px * pn;
+  a[25] = a[10];
+  a[22] = bb * r.py * pn;
+  a[27] = a[22];
+  a[28] = bb;
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j < 6; j++) b[i + 6 * j] += a[i + 6 * j];
+}
+
+// diff_str_kick.c:19-83 (bend = false), diff_bnd_kick.c:101-174 (bend = true)
+__device__ __forceinline__ void d_rad_kick(R6& r, cdouble* a, cdouble* bp, int n, double L,
+                                           double irho, double radc, double difc, bool bend,
+                                           double* b) {
+  double re = bp[n], im = a[n];
+  for (int k = n - 1; k >= 0; k--) {
+    const double t = re * r.x - im * r.y + bp[k];
+    im = im * r.x + re * r.y + a[k];
+    re = t;
+  }
+  double pn = 1.0 / (1.0 + r.dp);
+  const double x = r.x;
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  double b2p, factor;
+  if (bend) {
+    const double by = re + irho, bx = im;
+    const double hx = 1 + x * irho;
+    const double vn2 = 1.0 / ((1.0 + x * irho) * (1.0 + x * irho) + xpr * xpr + ypr * ypr);
+    b2p = ((by * hx) * (by * hx) + (bx * hx) * (bx * hx) + (bx * ypr - by * xpr) * (bx * ypr - by * xpr)) * vn2;
+    factor = (1.0 + x * irho + (xpr * xpr + ypr * ypr) / 2.0) / (pn * pn) * L;
+  } else {
+    const double vn2 = 1.0 / (1.0 + xpr * xpr + ypr * ypr);
+    const double cr = im * ypr - re * xpr;
+    b2p = (re * re + im * im + cr * cr) * vn2;
+    factor = (1.0 + (xpr * xpr + ypr * ypr) / 2.0) / (pn * pn) * L;
+  }
+  d_kick_m(r, a, bp, n, L, irho, bend, b);
+  d_kick_b(r, difc, b2p, factor, b);
+  const double d0 = r.dp;
+  r.dp -= radc * b2p * factor;
+  pn = 1.0 / (1.0 + r.dp);
+  r.px = xpr / pn;
+  r.py = ypr / pn;
+  if (bend) {
+    r.px -= L * (re - (d0 - r.x * irho) * irho);
+    r.py += L * im;
+    r.ct += L * irho * r.x;
+  } else {
+    r.px -= L * re;
+    r.py += L * im;
+  }
+}
+
+// diff_str_exactkick.c:14-71
+__device__ __forceinline__ void d_exact_rad_kick(R6& r, cdouble* a, cdouble* bp, int n, double L,
+                                                 double radc, double difc, double* b) {
+  double re = bp[n], im = a[n];
+  for (int k = n - 1; k >= 0; k--) {
+    const double t = re * r.x - im * r.y + bp[k];
+    im = im * r.x + re * r.y + a[k];
+    re = t;
+  }
+  double pn = 1.0 / (1.0 + r.dp);
+  const double xpr = r.px * pn;
+  const double ypr = r.py * pn;
+  const double t = im * xpr + re * ypr;
+  const double b2p = im * im + re * re - t * t;
+  const double factor = L / (pn * pn) / sqrt(1.0 - xpr * xpr - ypr * ypr);
+  d_kick_m(r, a, bp, n, L, 0.0, false, b);
+  d_kick_b(r, difc, b2p, factor, b);
+  r.dp -= radc * b2p * factor;
+  pn = 1.0 / (1.0 + r.dp);
+  r.px = xpr / pn;
+  r.py = ypr / pn;
+  r.px -= L * re;
+  r.py += L * im;
+}
+
+// diff_bend_fringe.c:2-53; mode and constants as for tracking
+__device__ __forceinline__ void d_edge(R6& r, int mode, double irho, double fx, double c0,
+                                       double edge_angle, double psi, double sgn, bool bterm,
+                                       double* b) {
+  const double pn = 1.0 / (1.0 + r.dp);
+  double fy;
+  if (mode == E_CONST)
+    fy = c0;
+  else if (mode == E_DIVP)
+    fy = c0 / (1.0 + r.dp);
+  else if (mode == E_MULPN)
+    fy = c0 * pn;
+  else
+    fy = edge_fy_tan(mode, r.dp, r.px, irho, edge_angle, psi, sgn);
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    b[1 + 6 * m] += fx * b[6 * m];
+    b[3 + 6 * m] -= fy * b[2 + 6 * m];
+  }
+  if (bterm)
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+      b[3 + 6 * m] -= b[4 + 6 * m] * r.y * (irho * irho + fy * fy) * psi * pn * pn / irho;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    b[m + 6 * 1] += fx * b[m + 6 * 0];
+    b[m + 6 * 3] -= fy * b[m + 6 * 2];
+  }
+  if (bterm)
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+      b[m + 6 * 3] -= b[m + 6 * 4] * r.y * (irho * irho + fy * fy) * psi * pn * pn / irho;
+  r.px += r.x * fx;
+  r.py -= r.y * fy;
+}
+
+// One radiating element from its entrance orbit: the path of
+// StrMPoleSymplectic4RadPass.c:61-110, BndMPoleSymplectic4RadPass.c:76-123
+// and ExactMultipoleRadPass.c:56-94 with the matrix updates.
+__device__ __noinline__ void diffusion_elem(R6 r, cDevElem& h, cdouble* pool, double* b) {
+  cdouble* s = pool + h.sc;
+  cdouble* a = pool + h.poly;
+  cdouble* bp = a + h.order + 1;
+  const int n = h.order;
+  const bool bend = h.kind == K_BNDRAD;
+  const bool exact = h.kind == K_EXMPOLERAD;
+  const double irho = s[S_IRHO];
+  const double radc = s[S_RADC], difc = s[S_DIFC];
+  const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
+  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  if (exact) {
+    if (h.fringe & 0xff) r = multipole_fringe(r, a, bp, n, 1.0);
+    for (int m = 0; m < h.nslice; m++) {
+      d_exact_drift(r, l1, b);
+      d_exact_rad_kick(r, a, bp, n, k1, radc, difc, b);
+      d_exact_drift(r, l2, b);
+      d_exact_rad_kick(r, a, bp, n, k2, radc, difc, b);
+      d_exact_drift(r, l2, b);
+      d_exact_rad_kick(r, a, bp, n, k1, radc, difc, b);
+      d_exact_drift(r, l1, b);
+    }
+    return;  // the exit maps do not change B
+  }
+  if (bend)
+    d_edge(r, h.edge & 0xff, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0,
+           (h.flags & F_EDGEB_IN) != 0, b);
+  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
+  for (int m = 0; m < h.nslice; m++) {
+    d_drift(r, l1, b);
+    d_rad_kick(r, a, bp, n, k1, irho, radc, difc, bend, b);
+    d_drift(r, l2, b);
+    d_rad_kick(r, a, bp, n, k2, irho, radc, difc, bend, b);
+    d_drift(r, l2, b);
+    d_rad_kick(r, a, bp, n, k1, irho, radc, difc, bend, b);
+    d_drift(r, l1, b);
+  }
+  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
+  if (bend)
+    d_edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT],
+           -1.0, (h.flags & F_EDGEB_OUT) != 0, b);
+}
+
+__global__ void __launch_bounds__(64) diffusion_kernel(DiffArgs args) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= args.nelem) return;
+  cDevElem& h = ((cDevElem*)args.elems)[e];
+  double b[36];
+#pragma unroll
+  for (int i = 0; i < 36; i++) b[i] = 0.0;
+  if (h.kind == K_STRRAD || h.kind == K_BNDRAD || h.kind == K_EXMPOLERAD) {
+    const double* o = args.orbits + 6 * (int64_t)e;
+    R6 r = {o[0], o[1], o[2], o[3], o[4], o[5]};
+    if (!__builtin_isnan(r.x)) diffusion_elem(r, h, (cdouble*)args.pool, b);
+  }
+  double* out = args.bdiff + 36 * (int64_t)e;
+#pragma unroll
+  for (int i = 0; i < 36; i++) out[i] = b[i];
+}
+
+#endif  // ATX_STRICT
+
+// EXT: the kernel variant that also runs CavityPass and the exact passes.
+// Lattices without them use the variant without, whose element switch keeps
+// the register demand of the common passes only.
+template <bool EXT>
 __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int nturn) {
+  if (EXT) {
+    switch (h.kind) {
+      case K_CAVITY: r = cavity(r, pool + h.sc, (h.flags & F_THICK) != 0); return;
+      case K_EXDRIFT:  // ExactDriftPass.c:15-44
+        entry_xform(r, h, pool);
+        apertures(r, h, pool);
+        exact_drift(r, pool[h.sc + S_D_LEN]);
+        r.ct -= pool[h.sc + S_D_LEN];
+        apertures(r, h, pool);
+        exit_xform(r, h, pool);
+        return;
+      case K_EXMPOLE: exact_mpole(r, h, pool, false); return;
+      case K_EXMPOLERAD: exact_mpole(r, h, pool, true); return;
+      default: break;
+    }
+  }
   switch (h.kind) {
     case K_IDENTITY:  // IdentityPass.c:20-43
       entry_xform(r, h, pool);
@@ -804,17 +1131,6 @@ __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int n
     case K_RF: rf_cavity(r, h, pool, nturn); break;
     case K_CORR: corrector(r, h, pool); break;
     case K_APERTURE: rect_ap(r, pool + h.rap); break;  // AperturePass.c:9-26
-    case K_CAVITY: r = cavity(r, pool + h.sc, (h.flags & F_THICK) != 0); break;
-    case K_EXDRIFT:  // ExactDriftPass.c:15-44
-      entry_xform(r, h, pool);
-      apertures(r, h, pool);
-      exact_drift(r, pool[h.sc + S_D_LEN]);
-      r.ct -= pool[h.sc + S_D_LEN];
-      apertures(r, h, pool);
-      exit_xform(r, h, pool);
-      break;
-    case K_EXMPOLE: exact_mpole(r, h, pool, false); break;
-    case K_EXMPOLERAD: exact_mpole(r, h, pool, true); break;
     default: break;
   }
 }
@@ -855,6 +1171,7 @@ __device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
   rp[5 * cs] = r.ct;
 }
 
+template <bool EXT>
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
   extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
@@ -984,7 +1301,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
         if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
       }
       cDevElem& h = elems[e];
-      if (!__builtin_isnan(r.x)) element(r, h, pool, nturn);
+      if (!__builtin_isnan(r.x)) element<EXT>(r, h, pool, nturn);
       if (check) {
         if (losses) {
           if (!lost && is_lost(r)) {
@@ -1012,8 +1329,22 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 }  // namespace atx
 
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
-                                 hipStream_t stream) {
-  hipLaunchKernelGGL(atx::ATX_NS::track_kernel, dim3(nblocks), dim3(block),
-                     atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
+                                 int ext, hipStream_t stream) {
+  if (ext)
+    hipLaunchKernelGGL(atx::ATX_NS::track_kernel<true>, dim3(nblocks), dim3(block),
+                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
+  else
+    hipLaunchKernelGGL(atx::ATX_NS::track_kernel<false>, dim3(nblocks), dim3(block),
+                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
   return hipGetLastError();
 }
+
+#if ATX_STRICT
+extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs* args, hipStream_t stream) {
+  const int block = 64;
+  const int nblocks = (args->nelem + block - 1) / block;
+  if (nblocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::ATX_NS::diffusion_kernel, dim3(nblocks), dim3(block), 0, stream, *args);
+  return hipGetLastError();
+}
+#endif

@@ -38,6 +38,7 @@ EXPORTS = (
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
     "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
+    "atx_diffusion_matrices",
 )
 
 
@@ -111,6 +112,8 @@ def load_library(path: str | None = None) -> C.CDLL:
                                     C.c_int32, C.c_void_p]
         lib.atx_element_set_array_at.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, _dp,
                                                  C.c_int64]
+        lib.atx_diffusion_matrices.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                               C.c_double, C.c_void_p, C.c_uint32]
         lib.atx_device_count.argtypes = [C.POINTER(C.c_int)]
         lib.atx_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.POINTER(C.c_int), C.c_char_p, C.c_int]
@@ -219,6 +222,19 @@ class DeviceLattice:
                                     float(energy), float(rest_energy), lam_ptr,
                                     slots.ctypes.data_as(ip), coeffs.ctypes.data_as(ip),
                                     grad_ptr, int(nparams), stream), self.lib)
+
+    def diffusion_matrices(self, orbits: np.ndarray, energy: float,
+                           rest_energy: float) -> np.ndarray:
+        """Diffusion matrix of every element from its entrance orbit
+        (atx_diffusion_matrices): orbits (6, nelem) -> (nelem, 6, 6)."""
+        o = np.asfortranarray(orbits, dtype=np.float64)
+        if o.shape != (6, self.nelem):
+            raise ValueError("orbits must have shape (6, number of elements)")
+        b = np.zeros(36 * self.nelem, dtype=np.float64)
+        _check(self.lib.atx_diffusion_matrices(self.h, o.ctypes.data, b.ctypes.data,
+                                               float(energy), float(rest_energy), None, 0),
+               self.lib)
+        return b.reshape(self.nelem, 6, 6).transpose(0, 2, 1).copy()
 
     def check(self, energy: float, rest_energy: float) -> None:
         _check(self.lib.atx_lattice_check(self.h, float(energy), float(rest_energy)), self.lib)

@@ -161,6 +161,22 @@ int atx_adjoint(atx_lattice *lat, const double *states, int32_t nturns, int32_t 
                 const int32_t *grad_slot, const int32_t *grad_coeff, double *grad,
                 int32_t num_params, void *stream);
 
+/* --- Diffusion matrices --------------------------------------------------
+ * Replaces at_diffmatrix (pyat/at.c:755-831), called once per element by
+ * ohmi_envelope (pyat/at/physics/radiation.py:78-121): for every element of
+ * the lattice at once, the Ohmi diffusion matrix B accumulated through the
+ * element by its radiating pass from the orbit at its entrance, with the
+ * reference's per-step propagation B <- M B M^T + D (diff_drift.c,
+ * diff_thinkick.c, diff_str_kick.c, diff_bnd_kick.c, diff_bend_fringe.c,
+ * diff_exactdrift.c, diff_str_exactkick.c). Elements without a radiating
+ * pass (StrMPole/BndMPole/ExactMultipole Rad) get zeros.
+ *   orbits  (6, nelem) F-order: orbit at the entrance of each element
+ *   bdiff   nelem blocks of 36 doubles, column-major 6x6 ([i][j] at i+6j)
+ * Host arrays (synchronous) or, with ATX_DEVICE_POINTERS, device arrays
+ * (asynchronous on stream). One lane per element, one launch. */
+int atx_diffusion_matrices(atx_lattice *lat, const double *orbits, double *bdiff, double energy,
+                           double rest_energy, void *stream, uint32_t flags);
+
 /* Replace an attribute of element `index` (e.g. PolynomB of a magnet whose
  * strength is a differentiation parameter); the compiled lattice is
  * rebuilt on the next call. No counterpart in the reference, which

@@ -237,3 +237,23 @@ def tangent_ref(element, r6) -> np.ndarray:
         raise RuntimeError(lib.err().decode())
     del keep
     return m
+
+
+def diffusion_ref(element, orbit, energy, particle=None) -> np.ndarray:
+    """Ohmi diffusion matrix (6x6) of one element from its entrance orbit,
+    computed by the reference cores as at_diffmatrix does (ref_diffusion
+    in ref_driver.c)."""
+    lib = _lib("ref")
+    fn = lib.lib.ref_diffusion
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(OracleElem), _dp, _dp, C.c_double, C.c_double]
+    keep: list = []
+    s = make_elem(element, keep)
+    o = np.ascontiguousarray(orbit, dtype=np.float64).reshape(6)
+    b = np.zeros(36, np.float64)
+    rest = float(getattr(particle, "rest_energy", 0.0)) if particle is not None else 0.0
+    rc = fn(C.byref(s), _ptr(o), _ptr(b), float(energy), rest)
+    if rc:
+        raise RuntimeError(lib.err().decode())
+    del keep
+    return b.reshape(6, 6).T.copy()

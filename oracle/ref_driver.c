@@ -414,6 +414,48 @@ int ref_tangent(const oracle_elem *e, const double *r6, double *m66)
     return 0;
 }
 
+/*
+ * Diffusion matrix of one element from the orbit at its entrance, as
+ * at_diffmatrix (pyat/at.c:755-831) computes it: the element's own pass
+ * (the radiating cores propagate bdiff; the others leave it at zero) on one
+ * particle with the ring energy and rest energy. bdiff: 36 doubles,
+ * column-major.
+ */
+int ref_diffusion(const oracle_elem *e, const double *orbit, double *bdiff, double energy,
+                  double rest_energy)
+{
+    rf_err[0] = 0;
+    for (int i = 0; i < 36; i++) bdiff[i] = 0.0;
+    if (e->pass != OR_STRMPOLESYMPLECTIC4RADPASS && e->pass != OR_BNDMPOLESYMPLECTIC4RADPASS &&
+        e->pass != OR_EXACTMULTIPOLERADPASS)
+        return 0;
+    rf_prep p;
+    int rc = rf_prepare(e, &p, energy, rest_energy, 0.0);
+    if (rc) return rc;
+    void *f = rf_resolve(e->pass);
+    if (!f) return rf_fail(4, rf_err);
+    double r[6];
+    memcpy(r, orbit, sizeof(r));
+    if (e->pass == OR_STRMPOLESYMPLECTIC4RADPASS)
+        ((fn_strrad)f)(r, e->Length, p.a, p.b, e->MaxOrder, e->NumIntSteps, e->FringeQuadEntrance,
+                       e->FringeQuadExit, D(e->fringeIntM0), D(e->fringeIntP0), D(e->T1), D(e->T2),
+                       D(e->R1), D(e->R2), D(e->RApertures), D(e->EApertures), D(e->KickAngle),
+                       e->FieldScaling, p.gamma, 1, bdiff);
+    else if (e->pass == OR_BNDMPOLESYMPLECTIC4RADPASS)
+        ((fn_bndrad)f)(r, e->Length, p.irho, p.a, p.b, e->MaxOrder, e->NumIntSteps,
+                       e->EntranceAngle, e->ExitAngle, e->FringeBendEntrance, e->FringeBendExit,
+                       e->FringeInt1, e->FringeInt2, e->FullGap, e->FringeQuadEntrance,
+                       e->FringeQuadExit, D(e->fringeIntM0), D(e->fringeIntP0), D(e->T1), D(e->T2),
+                       D(e->R1), D(e->R2), D(e->RApertures), D(e->EApertures), D(e->KickAngle),
+                       e->FieldScaling, p.gamma, 1, bdiff);
+    else
+        ((fn_exmpolerad)f)(r, e->Length, p.a, p.b, e->MaxOrder, e->NumIntSteps,
+                           e->FringeQuadEntrance, e->FringeQuadExit, D(e->T1), D(e->T2),
+                           D(e->R1), D(e->R2), D(e->RApertures), D(e->EApertures),
+                           D(e->KickAngle), e->FieldScaling, p.gamma, 1, bdiff);
+    return 0;
+}
+
 int ref_elempass(const oracle_elem *e, double *r_in, int32_t np, double energy,
                  double rest_energy)
 {
