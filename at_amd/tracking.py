@@ -22,7 +22,7 @@ from .lattice import AtError, Element, Lattice, Particle
 __all__ = [
     "MPMode", "lattice_track", "element_track", "internal_lpass", "internal_epass",
     "lattice_pass", "element_pass", "gpu_info", "gpu_core_count", "gpupass", "atpass",
-    "get_uint32_index", "End",
+    "get_uint32_index", "End", "diffusion_matrix", "diffusion_matrices",
 ]
 
 
@@ -272,6 +272,58 @@ def lattice_pass(lattice, r_in, nturns: int = 1, refpts=End, **kwargs):
 
 def element_pass(element, r_in, **kwargs):
     return element_track(element, r_in, in_place=True, **kwargs)
+
+
+def _diffusion_energy(energy, particle):
+    # at_diffmatrix takes the energy and particle only from its keywords
+    # (pyat/at.c:755-800): without a particle the rest energy stays 0 and
+    # the radiating cores take gamma as E / m_e (atelem.c:363-370)
+    if isinstance(particle, str):
+        particle = Particle(particle)
+    rest = float(getattr(particle, "rest_energy", 0.0)) if particle is not None else 0.0
+    return float(energy or 0.0), rest
+
+
+def diffusion_matrix(element: Element, rin, *, energy=None, particle=None) -> np.ndarray:
+    """Ohmi diffusion matrix of one element from its entrance orbit, with
+    the argument checks of at_diffmatrix (pyat/at.c:755-831): rin is a
+    Fortran-ordered (6, 1) double array. Non-radiating passes give zeros."""
+    if not isinstance(rin, np.ndarray):
+        raise TypeError("rin must be a numpy array")
+    if rin.shape[0] != 6:
+        raise ValueError("rin is not 6D")
+    if rin.dtype != np.float64:
+        raise ValueError("rin is not a double array")
+    if not rin.flags.f_contiguous:
+        raise ValueError("rin is not Fortran-aligned")
+    if rin.size != 6:
+        raise ValueError("Number of particles should be 1")
+    e, rest = _diffusion_energy(energy, particle)
+    dl = engine.DeviceLattice([element])
+    try:
+        return dl.diffusion_matrices(rin.reshape(6, 1), e, rest)[0]
+    finally:
+        dl.close()
+
+
+def diffusion_matrices(lattice, orbits, *, energy=None, particle=None,
+                       keep_lattice: bool = False) -> np.ndarray:
+    """Diffusion matrices of every element of a lattice in one launch:
+    orbits (6, nelem) holds each element's entrance orbit; returns
+    (nelem, 6, 6). Element by element this equals diffusion_matrix, which
+    pyat's _dmatr calls in a loop (physics/radiation.py:98-115). The device
+    lattice is shared with gpupass and reused with keep_lattice."""
+    if not isinstance(lattice, list):
+        lattice = list(lattice)
+    orbits = np.asfortranarray(orbits, dtype=np.float64)
+    e, rest = _diffusion_energy(energy, particle)
+    if not (keep_lattice and _Cache.lattice_id == id(lattice)
+            and _Cache.device_lattice is not None):
+        _Cache.device_lattice = None
+        _Cache.lattice_id = None
+        _Cache.device_lattice = engine.DeviceLattice(lattice)
+        _Cache.lattice_id = id(lattice)
+    return _Cache.device_lattice.diffusion_matrices(orbits, e, rest)
 
 
 def gpu_info() -> list:
