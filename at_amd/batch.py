@@ -50,12 +50,15 @@ def _check_particles(p):
 def track_batch(lattices, particles, nturns: int, *, turn: int = 0, energy: float,
                 rest_energy: float, losses: bool = True, loss_maps=None, block_size: int = 0,
                 strict: bool = False, stream=None, item_turns=None,
-                continue_losses: bool = False):
+                continue_losses: bool = False, repack: bool = False):
     """Track every (lattice, particles) pair for ``nturns`` in one launch,
     in place. ``item_turns``, if given, holds one (nturns, turn) pair per
     instance, which then replaces the common ``nturns``/``turn``. With
     ``continue_losses`` the given loss maps carry over from an earlier call on
-    the same particles. Returns the list of LossMap when ``losses``."""
+    the same particles. With ``repack`` the launch runs in turn blocks of 1,
+    1, 2, 4, ... turns and packs each instance's surviving particles into its
+    leading workgroups between blocks (one graph; identical results). Returns
+    the list of LossMap when ``losses``."""
     import torch
 
     if len(lattices) != len(particles):
@@ -79,5 +82,5 @@ def track_batch(lattices, particles, nturns: int, *, turn: int = 0, energy: floa
     s = stream if stream is not None else torch.cuda.current_stream()
     engine.track_batch(items, nturns, turn=turn, energy=energy, rest_energy=rest_energy,
                        losses=losses, block_size=block_size, stream=s.cuda_stream,
-                       strict=strict, continue_losses=continue_losses)
+                       strict=strict, continue_losses=continue_losses, repack=repack)
     return loss_maps if losses else None

@@ -135,6 +135,10 @@ struct Instance {
                             // 2: as 1, continuing the map of an earlier call
   int32_t elempass;         // 1: single element pass, no loss check (at.c at_elempass)
   int32_t pad;
+  // Repacked launches: lanes take particles idx[0..*nlive) instead of
+  // 0..npart (both written by repack_kernel); NULL for plain launches.
+  const int32_t* idx;
+  const int32_t* nlive;
 };
 
 // Per-element data of the adjoint kernel: gradient slot (-1: none) and

@@ -4,6 +4,7 @@
 // include/atx.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +20,7 @@
 extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
+extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, hipStream_t);
 
 namespace atx {
@@ -704,6 +706,148 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   return ATX_OK;
 }
 
+// Graph executables of repacked launches: launched asynchronously, kept
+// until the event recorded after them has completed, then destroyed.
+struct PendingGraph {
+  hipGraphExec_t exec;
+  hipEvent_t done;
+};
+std::mutex g_graph_mu;
+std::vector<PendingGraph> g_graphs;
+
+void reap_graphs() {
+  std::lock_guard<std::mutex> g(g_graph_mu);
+  size_t k = 0;
+  for (auto& pg : g_graphs) {
+    if (hipEventQuery(pg.done) == hipSuccess) {
+      (void)hipGraphExecDestroy(pg.exec);
+      (void)hipEventDestroy(pg.done);
+    } else {
+      g_graphs[k++] = pg;
+    }
+  }
+  g_graphs.resize(k);
+}
+
+// Turn-block boundaries of a repacked launch: 0, 1, 2, 4, ... < nturns, nturns.
+std::vector<int> repack_bounds(int nturns) {
+  std::vector<int> b{0};
+  for (int t = 1; t < nturns; t *= 2) b.push_back(t);
+  b.push_back(nturns);
+  return b;
+}
+
+// Batched launch in turn blocks with the survivors of every instance packed
+// into its leading workgroups between blocks (repack_kernel). All block
+// tables go up in one copy; the track and repack launches are captured on a
+// private stream into one graph, which runs on `stream`. Each particle's
+// arithmetic is unchanged, so results equal those of launch_instances.
+int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::vector<int32_t>*>& slots,
+                    int block, hipStream_t stream, bool strict, bool ext) {
+  if (block <= 0) block = kBlock;
+  if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
+  reap_graphs();
+  const size_t ninst = insts.size();
+  int nblocks = 0;
+  int maxt = 0;
+  int64_t nidx = 0;
+  for (auto& in : insts) {
+    in.block_begin = nblocks;
+    nblocks += (int)((in.npart + block - 1) / block);
+    maxt = std::max(maxt, (int)in.nturns);
+    nidx += in.npart;
+  }
+  if (nblocks == 0 || maxt == 0) return ATX_OK;
+  const std::vector<int> bounds = repack_bounds(maxt);
+  const int nblk = (int)bounds.size() - 1;
+  const size_t tab = ninst * sizeof(Instance);
+  size_t bytes = (size_t)nblk * tab;
+  std::vector<size_t> soff(ninst);
+  std::map<const std::vector<int32_t>*, size_t> placed;
+  std::vector<std::pair<size_t, const std::vector<int32_t>*>> copies;
+  for (size_t i = 0; i < ninst; i++) {
+    auto f = placed.find(slots[i]);
+    if (f != placed.end()) {
+      soff[i] = f->second;
+      continue;
+    }
+    soff[i] = bytes;
+    placed[slots[i]] = bytes;
+    copies.emplace_back(bytes, slots[i]);
+    bytes += slots[i]->size() * sizeof(int32_t);
+  }
+  const size_t upload = bytes;  // tables and slot tables; idx and nlive are written on the device
+  bytes = (bytes + 255) & ~(size_t)255;
+  const size_t idx_off = bytes;
+  bytes += (size_t)nidx * sizeof(int32_t);
+  const size_t nlive_off = bytes;
+  bytes += ninst * sizeof(int32_t);
+  char* d = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d, bytes, stream));
+  std::vector<char> host(upload);
+  for (auto& c : copies) std::memcpy(host.data() + c.first, c.second->data(), c.second->size() * sizeof(int32_t));
+  for (int k = 0; k < nblk; k++) {
+    int64_t ioff = 0;
+    for (size_t i = 0; i < ninst; i++) {
+      Instance in = insts[i];
+      in.slots = (const int32_t*)(d + soff[i]);
+      const int lo = bounds[k];
+      const int hi = std::min(bounds[k + 1], (int)insts[i].nturns);
+      in.nturns = std::max(0, hi - lo);
+      in.turn0 = insts[i].turn0 + lo;
+      if (k > 0) {
+        in.losses = insts[i].losses ? 2 : 0;
+        in.idx = (const int32_t*)(d + idx_off) + ioff;
+        in.nlive = (const int32_t*)(d + nlive_off) + i;
+      }
+      ioff += insts[i].npart;
+      std::memcpy(host.data() + (size_t)k * tab + i * sizeof(Instance), &in, sizeof(Instance));
+    }
+  }
+  hipError_t ce = hipMemcpyAsync(d, host.data(), upload, hipMemcpyHostToDevice, stream);
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  hipError_t ge = ce;
+  if (ge == hipSuccess) ge = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (ge == hipSuccess) ge = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+  if (ge == hipSuccess) {
+    hipError_t le = hipSuccess;
+    for (int k = 0; k < nblk && le == hipSuccess; k++) {
+      const Instance* t = (const Instance*)(d + (size_t)k * tab);
+      le = strict ? atx_launch_track_strict(t, (int)ninst, nblocks, block, ext ? 1 : 0, cap)
+                  : atx_launch_track_fast(t, (int)ninst, nblocks, block, ext ? 1 : 0, cap);
+      if (le == hipSuccess && k + 1 < nblk)
+        le = atx_launch_repack((const Instance*)(d + (size_t)(k + 1) * tab), (int)ninst, k == 0 ? 1 : 0, cap);
+    }
+    ge = hipStreamEndCapture(cap, &graph);
+    if (le != hipSuccess) ge = le;
+  }
+  if (ge == hipSuccess) ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (cap) (void)hipStreamDestroy(cap);
+  if (ge == hipSuccess) ge = hipGraphLaunch(exec, stream);
+  if (ge == hipSuccess) ge = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+  if (ge == hipSuccess) ge = hipEventRecord(done, stream);
+  if (ge == hipSuccess) {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    g_graphs.push_back({exec, done});
+  } else {
+    // nothing of the graph ran, or it has been queued before the event failed
+    if (done) (void)hipEventDestroy(done);
+    if (exec) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipGraphExecDestroy(exec);
+    }
+  }
+  hipError_t fe = hipFreeAsync(d, stream);
+  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (ge != hipSuccess) return fail(ATX_E_RUNTIME, std::string("repacked launch: ") + hipGetErrorString(ge));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  return ATX_OK;
+}
+
 // Device-side core of atx_track: all pointers are device pointers.
 int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
                  int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
@@ -1026,6 +1170,8 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     sorted_insts.push_back(insts[i]);
     sorted_slots.push_back(slots[i]);
   }
+  if (flags & ATX_REPACK)
+    return launch_repacked(sorted_insts, sorted_slots, block_size, stream, strict, ext);
   return launch_instances(sorted_insts, sorted_slots, block_size, stream, strict, ext);
 }
 

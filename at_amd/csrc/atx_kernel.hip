@@ -226,6 +226,21 @@ __device__ __forceinline__ double rcp1(double d) {
   return __builtin_fma(q, __builtin_fma(-d, q, 1.0), q);
 }
 
+// Fast build: reciprocal of 1 + delta after a kick from the one before it,
+// pn = 1/(1 + delta_before). The radiated fraction e = 1 - (1 + delta) pn
+// is about 1e-7 per kick; two Newton steps leave e^4 + an ulp. A wave in
+// which any lane radiated more than 1e-4 in one kick takes fast_rcp.
+__device__ __forceinline__ double rcp_after_kick(double opd2, double pn) {
+  const double e = __builtin_fma(-opd2, pn, 1.0);
+  double q = __builtin_fma(pn, e, pn);
+  q = __builtin_fma(q, __builtin_fma(-opd2, q, 1.0), q);
+  const bool far = __builtin_fabs(e) > 1e-4;
+  if (__builtin_expect(__ballot(far) != 0, 0)) {
+    if (far) q = fast_rcp(opd2);
+  }
+  return q;
+}
+
 // Fast build: inside a radiating magnet delta changes only in the kicks, so
 // pn = 1/(1+delta) is computed once at entry and then by each kick for the
 // radiated delta, and carried to the drifts.
@@ -272,12 +287,19 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
   const double s2 = __builtin_fma(xpr, xpr, ypr * ypr);
   const double cr = __builtin_fma(im, ypr, -re * xpr);
   const double b2 = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im));
-  const double g = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);  // (1+s2/2)/(1+s2)
+  // g = (1+s2/2)/(1+s2) = 1 - (s2/2)/(1+s2), with 1 - s2 + s2^2 for the
+  // reciprocal: relative error s2^4/2 in g, below 1e-16 for s2 <= 1e-4
+  // (angles up to 10 mrad); a wave with a wider angle divides.
+  double g = __builtin_fma(-0.5 * s2, __builtin_fma(s2, s2, 1.0 - s2), 1.0);
+  const bool wide = s2 > 1e-4;
+  if (__builtin_expect(__ballot(wide) != 0, 0)) {
+    if (wide) g = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);
+  }
   r.dp = __builtin_fma(-radk, (b2 * g) * (opd * opd), r.dp);
   const double opd2 = 1.0 + r.dp;
   r.px = __builtin_fma(xpr, opd2, -kl * re);
   r.py = __builtin_fma(ypr, opd2, kl * im);
-  pn_io = fast_rcp(opd2);
+  pn_io = rcp_after_kick(opd2, pn_in);
 #endif
 }
 
@@ -335,7 +357,7 @@ __device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int 
   r.px = __builtin_fma(xpr, opd2, -kl * v);
   r.py = __builtin_fma(ypr, opd2, kl * im);
   r.ct = __builtin_fma(kli, x, r.ct);
-  pn_io = fast_rcp(opd2);
+  pn_io = rcp_after_kick(opd2, pn_in);
 #endif
 }
 
@@ -1186,14 +1208,18 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   const int tid = (int)threadIdx.x;
   const int B = (int)blockDim.x;
   const int npart = I.npart;
-  const int first = (bid - I.block_begin) * B;  // first particle of this workgroup
+  // lanes of a repacked launch take the surviving particles idx[0..nlive)
+  const int32_t* idx = I.idx;
+  const int nslot = idx ? __builtin_amdgcn_readfirstlane(*I.nlive) : npart;
+  const int first = (bid - I.block_begin) * B;  // first slot of this workgroup
   // waves of this workgroup that hold particles (uniform); the others end now
-  int nw = (min(B, npart - first) + 63) >> 6;
+  int nw = (min(B, nslot - first) + 63) >> 6;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave >= nw) return;
 
   int p = first + tid;
-  bool has = p < npart;  // this lane holds a particle
+  bool has = p < nslot;  // this lane holds a particle
+  if (has && idx) p = idx[p];
   cDevElem* elems = (cDevElem*)I.elems;
   cdouble* pool = (cdouble*)I.pool;
   cint* slots = (cint*)I.slots;
@@ -1340,6 +1366,58 @@ extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nb
 }
 
 #if ATX_STRICT
+namespace atx {
+// Repack between the turn blocks of a batched launch: one workgroup per
+// instance lists, in order, the particles still alive (x not NaN: a lost
+// particle is set to NaN, at.c:142-161) in idx[0..*nlive). The first pass
+// scans 0..npart, later passes the previous list, in place: a chunk is read
+// completely before it is written, and writes never pass the read front.
+__global__ void __launch_bounds__(1024) repack_kernel(const Instance* __restrict__ insts,
+                                                      int first_pass) {
+  __shared__ int s_wave[16];
+  const Instance& I = insts[blockIdx.x];
+  int32_t* idx = const_cast<int32_t*>(I.idx);
+  int32_t* nlive = const_cast<int32_t*>(I.nlive);
+  if (!idx) return;
+  const int n = first_pass ? I.npart : *nlive;
+  const int tid = (int)threadIdx.x;
+  const int wave = tid >> 6;
+  const int nwaves = (int)blockDim.x >> 6;
+  int out = 0;
+  for (int base = 0; base < n; base += (int)blockDim.x) {
+    const int s = base + tid;
+    int q = 0;
+    bool alive = false;
+    if (s < n) {
+      q = first_pass ? s : idx[s];
+      alive = !__builtin_isnan(I.r[(int64_t)q * I.pstride]);
+    }
+    const uint64_t m = __ballot(alive);
+    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    if ((tid & 63) == 0) s_wave[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, total = 0;
+    for (int k = 0; k < nwaves; k++) {
+      const int c = s_wave[k];
+      off += (k < wave) ? c : 0;
+      total += c;
+    }
+    if (alive) idx[out + off + rank] = q;
+    out += total;
+    __syncthreads();
+  }
+  if (tid == 0) *nlive = out;
+}
+}  // namespace atx
+
+extern "C" hipError_t atx_launch_repack(const atx::Instance* d_insts, int ninst, int first_pass,
+                                        hipStream_t stream) {
+  if (ninst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::repack_kernel, dim3(ninst), dim3(1024), 0, stream, d_insts, first_pass);
+  return hipGetLastError();
+}
+
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs* args, hipStream_t stream) {
   const int block = 64;
   const int nblocks = (args->nelem + block - 1) / block;

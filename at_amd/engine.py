@@ -30,6 +30,7 @@ ATX_STRICT = 0x2
 ATX_KEEP_COUNTER = 0x4
 ATX_SOA = 0x8
 ATX_CONTINUE_LOSSES = 0x10
+ATX_REPACK = 0x20
 
 # Exported symbols, as declared in include/atx.h.
 EXPORTS = (
@@ -308,13 +309,14 @@ class DeviceLattice:
 def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
                 rest_energy: float = 0.0, losses: bool = True, block_size: int = 0,
                 stream: int | None = None, strict: bool = False, soa: bool = False,
-                continue_losses: bool = False) -> None:
+                continue_losses: bool = False, repack: bool = False) -> None:
     """One launch over many (DeviceLattice, device particle buffer) pairs.
 
     items: sequence of dicts with keys ``lattice`` (DeviceLattice), ``r``
     (device pointer), ``n`` (particles), with losses ``lost`` = (flag,
     turn, elem, coord) device pointers, and optionally ``nturns``/``turn``
-    for an item that tracks its own number of turns."""
+    for an item that tracks its own number of turns. ``repack``: turn blocks
+    with the survivors packed between them, run as one graph (ATX_REPACK)."""
     lib = load_library()
     arr = (BatchItem * max(len(items), 1))()
     for i, it in enumerate(items):
@@ -327,7 +329,7 @@ def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
         if losses:
             b.lost_flag, b.lost_turn, b.lost_elem, b.lost_coord = it["lost"]
     flags = ((ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
-             | (ATX_CONTINUE_LOSSES if continue_losses else 0))
+             | (ATX_CONTINUE_LOSSES if continue_losses else 0) | (ATX_REPACK if repack else 0))
     _check(lib.atx_track_batch(arr, len(items), int(nturns), int(turn), float(energy),
                                float(rest_energy), int(bool(losses)), int(block_size), stream,
                                flags), lib)

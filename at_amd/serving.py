@@ -92,11 +92,14 @@ class Server:
     hbm_budget: bytes of particle state and loss maps admitted at once.
     max_active: most requests in one launch.
     turn_block: turns per launch (admission granularity).
-    depth: launches queued ahead of the device."""
+    depth: launches queued ahead of the device.
+    repack: each launch runs in turn blocks with the survivors packed
+    between them (ATX_REPACK; identical results)."""
 
     def __init__(self, lattices, *, energy: float, rest_energy: float,
                  hbm_budget: int = 64 << 30, max_active: int = 256, turn_block: int = 128,
-                 block_size: int = 0, depth: int = 2, strict: bool = False):
+                 block_size: int = 0, depth: int = 2, strict: bool = False,
+                 repack: bool = True):
         import torch
 
         self.dls = [l if isinstance(l, engine.DeviceLattice) else engine.DeviceLattice(l)
@@ -109,6 +112,7 @@ class Server:
         self.block_size = int(block_size)
         self.depth = max(1, int(depth))
         self.strict = bool(strict)
+        self.repack = bool(repack)
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.stream = torch.cuda.Stream(device=self.device)      # launches
         self.copy_stream = torch.cuda.Stream(device=self.device)  # device-to-host results
@@ -147,7 +151,7 @@ class Server:
             s.done_turns += nt
         engine.track_batch(items, 0, energy=self.energy, rest_energy=self.rest, losses=True,
                            block_size=self.block_size, stream=self.stream.cuda_stream,
-                           strict=self.strict, continue_losses=True)
+                           strict=self.strict, continue_losses=True, repack=self.repack)
         self.launches += 1
 
     def _copy_back(self, s: _Slot):

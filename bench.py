@@ -111,11 +111,24 @@ def bench_batch(args) -> dict:
     t0 = time.perf_counter()
     e0.record(stream)
     lms = track_batch(dls, parts, T, energy=energy, rest_energy=rest, losses=True,
-                      block_size=block)
+                      block_size=block, repack=not args.no_repack)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
+    # the same launch without repacking between turn blocks
+    plain_ms = plain_same = None
+    if not args.no_repack and not args.no_compare:
+        parts3 = [b.clone() for b in base]
+        (p0, p1), = _events(1)
+        torch.cuda.synchronize()
+        p0.record(stream)
+        track_batch(dls, parts3, T, energy=energy, rest_energy=rest, losses=True, block_size=block)
+        p1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = p0.elapsed_time(p1)
+        plain_same = all(torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0))
+                         for a, b in zip(parts, parts3))
     flops = passes = 0.0
     for lat, lm in zip(lats, lms):
         fl, pa = tracked_flops_and_passes(lat, T, 0, lm.flag.cpu().numpy().astype(bool),
@@ -149,10 +162,14 @@ def bench_batch(args) -> dict:
         "config": {"workload": f"{len(dls)} error realisations of large_ring ({len(lats[0])} "
                                f"elements; rolls 100 urad, gradients 1e-3, residual shifts "
                                f"3 um), DA polar grids of U[1000, 40000] particles "
-                               f"({ntot} total), {T} turns, one launch",
+                               f"({ntot} total), {T} turns, one launch"
+                               + ("" if args.no_repack else
+                                  " (turn blocks with survivors repacked, one graph)"),
                    "block_size": block, "autotune_ms": tune["ms"], "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+        "plain_launch_ms": plain_ms,
+        "identical_to_plain_launch": plain_same,
         "one_launch_per_instance_ms": seq_ms,
         "speedup_vs_one_launch_per_instance": (seq_ms / ms) if seq_ms else None,
         "identical_to_one_launch_per_instance": same if seq_ms else None,
@@ -182,7 +199,8 @@ def bench_serve(args) -> dict:
         q.instance = idx[q.instance]
     dl_list = [dls[i] for i in used]
     energy, rest = base.energy, base.particle.rest_energy
-    srv = Server(dl_list, energy=energy, rest_energy=rest, turn_block=args.turn_block)
+    srv = Server(dl_list, energy=energy, rest_energy=rest, turn_block=args.turn_block,
+                 repack=not args.no_repack)
     # warm-up: one small request through each path
     from at_amd.serving import Request
     warm = [Request(0, 0, trace[0].particles[:, :64], 2)]
@@ -317,6 +335,8 @@ def main() -> int:
                          "trace; adjoint: strength gradients")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-turns", type=int, default=256)
+    ap.add_argument("--no-repack", action="store_true",
+                    help="batch/serve: one plain launch instead of repacked turn blocks")
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--turn-block", type=int, default=128)

@@ -10,7 +10,7 @@ import pytest
 
 from at_amd import engine
 from at_amd.autotune import tune_batch
-from at_amd.batch import track_batch
+from at_amd.batch import LossMap, track_batch
 from at_amd.workloads import da_polar_grid, error_realisation, large_ring
 from oracle import pyoracle as po
 
@@ -72,3 +72,71 @@ def test_autotuner_picks_a_candidate_and_keeps_state(tmp_path, monkeypatch):
     assert set(res["ms"]) == {"64", "128", "256", "512"}
     for a, b in zip(tens, before):
         assert torch.equal(a, b)
+
+
+def _lm_arrays(lm):
+    return (lm.flag.cpu().numpy().astype(bool), lm.turn.cpu().numpy().view(np.uint32),
+            lm.elem.cpu().numpy().view(np.uint32), lm.coord.cpu().numpy())
+
+
+@pytest.mark.parametrize("losses", [True, False], ids=["losses", "nolosses"])
+@pytest.mark.parametrize("block", [64, 256])
+def test_repacked_launch_equals_plain(losses, block):
+    """Turn blocks with the survivors repacked between them (one graph):
+    coordinates and loss maps equal the plain launch bit for bit, on every
+    call (the graph is rebuilt and replayed per call), with per-item turn
+    counts that end in different blocks."""
+    import torch
+
+    base = large_ring(ncells=2, cavity=True)
+    lats = [error_realisation(base, 300 + i) for i in range(4)]
+    dls = [engine.DeviceLattice(l) for l in lats]
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dev = torch.device("cuda:0")
+    sizes = [2000, 700, 1500, 333]
+    parts = [da_polar_grid(n, xmax=12e-3, ymax=5e-3) for n in sizes]
+    turns = [(37, 0), (64, 5), (9, 1), (1, 0)]
+    plain = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    lm_plain = track_batch(dls, plain, 0, energy=energy, rest_energy=rest, losses=losses,
+                           block_size=block, item_turns=turns)
+    torch.cuda.synchronize()
+    nlost = 0
+    for call in range(3):
+        rep = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+        lm_rep = track_batch(dls, rep, 0, energy=energy, rest_energy=rest, losses=losses,
+                             block_size=block, item_turns=turns, repack=True)
+        # eager work between the graph runs
+        torch.cuda.synchronize()
+        for i in range(len(parts)):
+            a, b = rep[i].cpu().numpy(), plain[i].cpu().numpy()
+            np.testing.assert_array_equal(a, b, err_msg=f"call {call} item {i}")
+            if losses:
+                for x, y in zip(_lm_arrays(lm_rep[i]), _lm_arrays(lm_plain[i])):
+                    np.testing.assert_array_equal(x, y, err_msg=f"call {call} item {i}")
+                nlost += int(lm_rep[i].flag.sum())
+            else:
+                nlost += int(np.isnan(a[:, 0]).sum())
+    assert nlost > 0
+
+
+def test_repacked_launch_continues_losses():
+    """Two repacked calls with ATX_CONTINUE_LOSSES equal one plain call."""
+    import torch
+
+    lats = [error_realisation(large_ring(ncells=2, cavity=True), 410 + i) for i in range(2)]
+    dls = [engine.DeviceLattice(l) for l in lats]
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dev = torch.device("cuda:0")
+    parts = [da_polar_grid(900, xmax=12e-3, ymax=5e-3), da_polar_grid(450, xmax=12e-3, ymax=5e-3)]
+    one = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    lm_one = track_batch(dls, one, 20, energy=energy, rest_energy=rest)
+    two = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    lm_two = [LossMap.empty(p.shape[1], dev) for p in parts]
+    track_batch(dls, two, 6, energy=energy, rest_energy=rest, loss_maps=lm_two, repack=True)
+    track_batch(dls, two, 14, turn=6, energy=energy, rest_energy=rest, loss_maps=lm_two,
+                continue_losses=True, repack=True)
+    torch.cuda.synchronize()
+    for i in range(2):
+        np.testing.assert_array_equal(two[i].cpu().numpy(), one[i].cpu().numpy())
+        for x, y in zip(_lm_arrays(lm_two[i]), _lm_arrays(lm_one[i])):
+            np.testing.assert_array_equal(x, y)

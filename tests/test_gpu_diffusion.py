@@ -41,10 +41,12 @@ def test_element_matches_reference(name, elem, energy):
     rest = ring.particle.rest_energy
     rng = np.random.default_rng(11)
     for k in range(4):
+        # the on-axis orbit gives zero diffusion in a pure quadrupole
         orbit = rng.normal(0.0, [1e-3, 1e-4, 5e-4, 5e-5, 2e-3, 1e-3]) if k else np.zeros(6)
         got = dl.diffusion_matrices(orbit.reshape(6, 1), energy, rest)[0]
         ref = po.diffusion_ref(elem, orbit, energy, ring.particle)
-        assert np.abs(ref).max() > 0.0
+        if k:
+            assert np.abs(ref).max() > 0.0
         _compare(got, ref, name not in TAN, f"{name} orbit {k}")
 
 
@@ -75,7 +77,7 @@ def test_ring_from_tracked_orbit():
             _compare(got[i], ref, True, f"element {i} {e.PassMethod}")
         else:
             assert np.all(got[i] == 0.0) and np.all(ref == 0.0)
-    assert nrad > 100
+    assert nrad == 92  # 4 cells of 7 bends and 16 quadrupoles
 
 
 def test_device_pointers_and_shape_errors():

@@ -45,14 +45,15 @@ def _trace(nreq=10, seed=2):
     return out
 
 
+@pytest.mark.parametrize("repack", [True, False], ids=["repack", "plain"])
 @pytest.mark.parametrize("realtime", [False, True], ids=["queued", "arrivals"])
-def test_served_results_equal_alone(realtime):
+def test_served_results_equal_alone(realtime, repack):
     lats = _instances()
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
     dls = [engine.DeviceLattice(l) for l in lats]
     trace = _trace()
     srv = Server(dls, energy=energy, rest_energy=rest, turn_block=8, max_active=4,
-                 hbm_budget=6000 * 105)
+                 hbm_budget=6000 * 105, repack=repack)
     res, st = srv.run(trace, realtime=realtime)
     print(st, flush=True)
     assert st["launches"] > 1
