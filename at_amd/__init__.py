@@ -12,6 +12,10 @@ from .lattice import (  # noqa: F401
     Corrector, Lattice, load_mat, e_mass, clight,
 )
 
+from .tracking import (  # noqa: E402,F401
+    End, MPMode, atpass, diffusion_matrix, element_pass, element_track, gpu_core_count, gpu_info,
+    internal_epass, internal_lpass, internal_plpass, lattice_pass, lattice_track, patpass,
+)
 from . import acceptance as _acceptance  # noqa: E402
 from . import physics as _physics  # noqa: E402
 from . import envelope as _envelope  # noqa: E402

@@ -22,7 +22,8 @@ from .lattice import AtError, Element, Lattice, Particle
 __all__ = [
     "MPMode", "lattice_track", "element_track", "internal_lpass", "internal_epass",
     "lattice_pass", "element_pass", "gpu_info", "gpu_core_count", "gpupass", "atpass",
-    "get_uint32_index", "End", "diffusion_matrix", "diffusion_matrices",
+    "get_uint32_index", "End", "diffusion_matrix", "diffusion_matrices", "patpass",
+    "internal_plpass",
 ]
 
 
@@ -206,6 +207,23 @@ def _lattice_pass(lattice: list, r_in, nturns: int = 1, refpts=End, no_varelem: 
     return gpupass(lattice, r_in, nturns, refpts=refs, **kwargs)
 
 
+@fortran_align
+def _plattice_pass(lattice: list, r_in, nturns: int = 1, refpts=End, seed: int | None = None,
+                   pool_size: int | None = None, start_method: str | None = None, **kwargs):
+    """internal_plpass (track.py:133-180). The reference splits the particles
+    over a process pool; one device launch tracks them all here, so
+    pool_size and start_method are accepted and unused."""
+    return _lattice_pass(lattice, r_in, nturns, refpts, seed=seed, **kwargs)
+
+
+def patpass(lattice, r_in, nturns: int = 1, refpts=End, pool_size: int | None = None,
+            start_method: str | None = None, **kwargs):
+    """Parallel lattice_pass (tracking/deprecated.py:190-226): r_in is
+    tracked in place and rout is returned."""
+    return _plattice_pass(lattice, r_in, nturns=nturns, refpts=refpts, pool_size=pool_size,
+                          start_method=start_method, **kwargs)
+
+
 _mp_table = {
     False: (False, False),
     True: (True, False),
@@ -352,5 +370,6 @@ def gpu_core_count(gpuPool: list[int] | None) -> int:
 
 internal_lpass = _lattice_pass
 internal_epass = _element_pass
+internal_plpass = _plattice_pass
 Lattice.track = lattice_track
 Element.track = element_track

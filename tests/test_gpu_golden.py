@@ -217,3 +217,27 @@ def test_gpu_info():
     assert len(info) >= 1
     name, arch, cus, platform = info[0]
     assert arch.startswith("gfx") and cus > 0 and platform.startswith("HIP")
+
+
+@pytest.mark.parametrize("func", ("lattice_track", "patpass", "internal_plpass"))
+def test_patpass_multiple_particles_and_turns(func):
+    """pyat/test/test_patpass.py:11-37 (Matlab values): the parallel entry
+    points give the same (6, N, 1, T) output as lattice_track."""
+    import at_amd
+
+    nturns, nparticles = 10, 10
+    rin = np.zeros((6, nparticles))
+    lattice = [at_amd.Drift("drift", 1.0)]
+    rin[1, 0] = 1e-6
+    rin[3, 0] = -2e-6
+    if func == "lattice_track":
+        rout, *_ = at_amd.lattice_track(lattice, rin, nturns, use_mp=True)
+    else:
+        rout = getattr(at_amd, func)(lattice, rin, nturns)
+    assert rout.shape == (6, nparticles, 1, nturns)
+    expected = np.array([1e-6, 1e-6, -2e-6, -2e-6, 0, 2.5e-12]).reshape(6, 1)
+    np.testing.assert_equal(rout[:, 0, :, 0], expected)
+    for i in range(nturns):
+        assert np.any(rout[:, 0, 0, i] != 0.0)
+        for j in range(1, nparticles):
+            np.testing.assert_equal(rout[:, j, 0, i], np.zeros(6))
