@@ -16,7 +16,14 @@ fixtures next to this script:
   hmba.npz       the reference fixture machine_data/hmba.mat: partial-turn
                  tracking (start_elem=100; start_elem=100, end_elem=110)
                  with the inputs of pyat/test/test_atpass.py:157-171, and
-                 8 particles x 5 turns with refpts.
+                 8 particles x 5 turns with refpts;
+  large_ring_1000.npz  the large ring, 2048 particles of the sigma x 10
+                 beam, 1000 turns with losses, and the chaotic mask: the
+                 particles whose loss record or final coordinates (by more
+                 than 1e-9) change when the inputs are scaled by
+                 1 +- 4e-16 or 1 +- 2e-15 (written only when asked for by
+                 name, `make_golden.py large_ring_1000`: about half an hour
+                 on 8 cores).
 
 Only this script runs reference code; the fixtures are data.
 """
@@ -84,6 +91,30 @@ def make_large_ring():
                         nturns=3, **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
 
 
+def _records(lm):
+    return np.stack([np.asarray(lm["islost"]).astype(np.int64), np.asarray(lm["turn"]).astype(np.int64),
+                     np.asarray(lm["elem"]).astype(np.int64)])
+
+
+def make_large_ring_1000(nthreads: int):
+    ring = large_ring()
+    r0 = gaussian_beam(2048, seed=31, scale=10.0)
+    r = r0.copy(order="F")
+    _, lm = po.atpass(ring, r, 1000, losses=True, impl="ref", nthreads=nthreads)
+    rec = _records(lm)
+    chaotic = np.zeros(r0.shape[1], bool)
+    for eps in (4e-16, -4e-16, 2e-15, -2e-15):
+        rp = np.asfortranarray(r0 * (1.0 + eps))
+        _, lp = po.atpass(ring, rp, 1000, losses=True, impl="ref", nthreads=nthreads)
+        surv = ~np.asarray(lm["islost"]) & ~np.asarray(lp["islost"])
+        chaotic |= np.any(_records(lp) != rec, axis=0)
+        chaotic[surv] |= np.max(np.abs(r[:, surv] - rp[:, surv]), axis=0) > 1e-9
+        print(f"perturbation {eps:g}: chaotic so far {int(chaotic.sum())}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "large_ring_1000.npz"), rin=r0, rout_final=r,
+                        nturns=1000, chaotic=chaotic,
+                        **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
+
+
 def make_hmba(mat):
     ring = load_mat(mat)
     out = {}
@@ -105,8 +136,11 @@ def make_hmba(mat):
 if __name__ == "__main__":
     if not po.available("ref"):
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` with /root/reference present")
-    make_zoo()
-    make_fodo()
-    make_large_ring()
-    make_hmba(os.path.join(HERE, "hmba.mat"))
+    if sys.argv[1:] == ["large_ring_1000"]:
+        make_large_ring_1000(min(16, len(os.sched_getaffinity(0))))
+    else:
+        make_zoo()
+        make_fodo()
+        make_large_ring()
+        make_hmba(os.path.join(HERE, "hmba.mat"))
     print("golden fixtures written to", HERE)

@@ -69,6 +69,25 @@ def test_golden_large_ring():
         np.testing.assert_array_equal(np.asarray(lm[k]), g[f"loss_{k}"])
 
 
+def test_golden_large_ring_1000_turns_subset():
+    """1000 turns of the large ring: the oracle repeats the reference bit for
+    bit on chaotic and regular particles alike (same operation order), on a
+    subset of the fixture's particles: lost and surviving ones of both
+    kinds."""
+    g = np.load(os.path.join(GOLD, "large_ring_1000.npz"))
+    lost, chaotic = g["loss_islost"], g["chaotic"]
+    pick = np.concatenate([np.flatnonzero(lost & chaotic)[:6], np.flatnonzero(lost & ~chaotic)[:6],
+                           np.flatnonzero(~lost & chaotic)[:6], np.flatnonzero(~lost & ~chaotic)[:6]])
+    assert pick.size >= 18
+    r = np.asfortranarray(g["rin"][:, pick])
+    _, lm = po.atpass(large_ring(), r, int(g["nturns"]), losses=True,
+                      nthreads=min(8, len(os.sched_getaffinity(0))))
+    np.testing.assert_array_equal(r, g["rout_final"][:, pick])
+    for k in ("islost", "turn", "elem"):
+        np.testing.assert_array_equal(np.asarray(lm[k]), g[f"loss_{k}"][pick])
+    np.testing.assert_array_equal(np.asarray(lm["coord"]), g["loss_coord"][:, pick])
+
+
 def test_golden_hmba_and_reference_values():
     g = np.load(os.path.join(GOLD, "hmba.npz"))
     ring = load_mat(os.path.join(GOLD, "hmba.mat"))
