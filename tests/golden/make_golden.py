@@ -20,10 +20,12 @@ fixtures next to this script:
   large_ring_1000.npz  the large ring, 2048 particles of the sigma x 10
                  beam, 1000 turns with losses, and the chaotic mask: the
                  particles whose loss record or final coordinates (by more
-                 than 1e-9) change when the inputs are scaled by
-                 1 +- 4e-16 or 1 +- 2e-15 (written only when asked for by
-                 name, `make_golden.py large_ring_1000`: about half an hour
-                 on 8 cores).
+                 than 1e-9) change when the inputs are scaled by 1 + eps,
+                 eps in PERTURBATIONS (up to 1e-13 relative: the size of the
+                 rounding differences that FMA contraction accumulates along
+                 a regular orbit in a few hundred turns); written only when
+                 asked for by name, `make_golden.py large_ring_1000` (about
+                 an hour on 8 cores).
 
 Only this script runs reference code; the fixtures are data.
 """
@@ -46,6 +48,7 @@ from at_amd.lattice import Lattice, Particle, load_mat  # noqa: E402
 from at_amd.workloads import fodo_ring, gaussian_beam, large_ring  # noqa: E402
 
 REFS3 = np.array([0, 1, 2], np.uint32)
+PERTURBATIONS = (4e-16, -4e-16, 2e-15, -2e-15, 1e-14, -1e-14, 1e-13, -1e-13)
 
 
 def make_zoo():
@@ -103,7 +106,7 @@ def make_large_ring_1000(nthreads: int):
     _, lm = po.atpass(ring, r, 1000, losses=True, impl="ref", nthreads=nthreads)
     rec = _records(lm)
     chaotic = np.zeros(r0.shape[1], bool)
-    for eps in (4e-16, -4e-16, 2e-15, -2e-15):
+    for eps in PERTURBATIONS:
         rp = np.asfortranarray(r0 * (1.0 + eps))
         _, lp = po.atpass(ring, rp, 1000, losses=True, impl="ref", nthreads=nthreads)
         surv = ~np.asarray(lm["islost"]) & ~np.asarray(lp["islost"])
@@ -111,7 +114,7 @@ def make_large_ring_1000(nthreads: int):
         chaotic[surv] |= np.max(np.abs(r[:, surv] - rp[:, surv]), axis=0) > 1e-9
         print(f"perturbation {eps:g}: chaotic so far {int(chaotic.sum())}", flush=True)
     np.savez_compressed(os.path.join(HERE, "large_ring_1000.npz"), rin=r0, rout_final=r,
-                        nturns=1000, chaotic=chaotic,
+                        nturns=1000, chaotic=chaotic, perturbations=np.array(PERTURBATIONS),
                         **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
 
 
