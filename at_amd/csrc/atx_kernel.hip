@@ -148,22 +148,44 @@ __device__ __forceinline__ void fastdrift(R6& r, double nl, double h2) {
 #endif
 }
 
-// Horner evaluation of sum (B_k + i A_k)(x + i y)^k (driftkick.c:92-110).
-// N >= 0: order known at compile time; N < 0: runtime order n.
+// Addends of the first Horner step, B[order-1] and A[order-1] (0 for order
+// 0), taken once per magnet. That step multiplies the constants B[order],
+// A[order] by a coordinate and adds another constant, and an FMA reads at
+// most one SGPR: the fast build therefore keeps these two addends in VGPRs
+// for the whole magnet instead of moving them from SGPRs at every kick.
 template <int N>
-__device__ __forceinline__ void field(const R6& r, cdouble* a, cdouble* b, int n, double& re,
-                                      double& im) {
+__device__ __forceinline__ void horner_addends(cdouble* a, cdouble* b, int n, double& blo,
+                                               double& alo) {
+  const int order = (N >= 0) ? N : n;
+  blo = (order >= 1) ? b[order - 1] : 0.0;
+  alo = (order >= 1) ? a[order - 1] : 0.0;
+#if !ATX_STRICT
+  if (N >= 1) {
+    asm("" : "=v"(blo) : "0"(blo));
+    asm("" : "=v"(alo) : "0"(alo));
+  }
+#endif
+}
+
+// Horner evaluation of sum (B_k + i A_k)(x + i y)^k (driftkick.c:92-110).
+// N >= 0: order known at compile time; N < 0: runtime order n. blo, alo:
+// B[order-1], A[order-1] from horner_addends (the same values).
+template <int N>
+__device__ __forceinline__ void field(const R6& r, cdouble* a, cdouble* b, int n, double blo,
+                                      double alo, double& re, double& im) {
   const int order = (N >= 0) ? N : n;
   re = b[order];
   im = a[order];
 #pragma unroll
   for (int k = order - 1; k >= 0; k--) {
+    const double bk = (N >= 1 && k == order - 1) ? blo : b[k];
+    const double ak = (N >= 1 && k == order - 1) ? alo : a[k];
 #if ATX_STRICT
-    const double t = re * r.x - im * r.y + b[k];
-    im = im * r.x + re * r.y + a[k];
+    const double t = re * r.x - im * r.y + bk;
+    im = im * r.x + re * r.y + ak;
 #else
-    const double t = __builtin_fma(re, r.x, __builtin_fma(-im, r.y, b[k]));
-    im = __builtin_fma(im, r.x, __builtin_fma(re, r.y, a[k]));
+    const double t = __builtin_fma(re, r.x, __builtin_fma(-im, r.y, bk));
+    im = __builtin_fma(im, r.x, __builtin_fma(re, r.y, ak));
 #endif
     re = t;
   }
@@ -171,19 +193,20 @@ __device__ __forceinline__ void field(const R6& r, cdouble* a, cdouble* b, int n
 
 // driftkick.c:92-115
 template <int N>
-__device__ __forceinline__ void str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl) {
+__device__ __forceinline__ void str_kick(R6& r, cdouble* a, cdouble* b, int n, double blo,
+                                         double alo, double kl) {
   double re, im;
-  field<N>(r, a, b, n, re, im);
+  field<N>(r, a, b, n, blo, alo, re, im);
   r.px -= kl * re;
   r.py += kl * im;
 }
 
 // driftkick.c:51-89; kli = kl*irho prepared on the host
 template <int N>
-__device__ __forceinline__ void bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                         double irho, double kli) {
+__device__ __forceinline__ void bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double blo,
+                                         double alo, double kl, double irho, double kli) {
   double re, im;
-  field<N>(r, a, b, n, re, im);
+  field<N>(r, a, b, n, blo, alo, re, im);
 #if ATX_STRICT
   r.px -= kl * (re - (r.dp - r.x * irho) * irho);
   r.py += kl * im;
@@ -258,10 +281,11 @@ __device__ __forceinline__ void rad_drift_pn(R6& r, double len, double pn) {
 // takes pn = 1/(1+delta) at entry and returns it updated to the radiated
 // delta.
 template <int N>
-__device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                             double radc, double radk, double& pn_io) {
+__device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int n, double blo,
+                                             double alo, double kl, double radc, double radk,
+                                             double& pn_io) {
   double re, im;
-  field<N>(r, a, b, n, re, im);
+  field<N>(r, a, b, n, blo, alo, re, im);
 #if ATX_STRICT
   (void)pn_io;
   (void)radk;
@@ -305,11 +329,11 @@ __device__ __forceinline__ void rad_str_kick(R6& r, cdouble* a, cdouble* b, int 
 
 // Radiating curved-frame kick (diff_bnd_kick.c:85-174, bdiff == NULL).
 template <int N>
-__device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
-                                             double irho, double kli, double radc, double radk,
-                                             double& pn_io) {
+__device__ __forceinline__ void rad_bnd_kick(R6& r, cdouble* a, cdouble* b, int n, double blo,
+                                             double alo, double kl, double irho, double kli,
+                                             double radc, double radk, double& pn_io) {
   double re, im;
-  field<N>(r, a, b, n, re, im);
+  field<N>(r, a, b, n, blo, alo, re, im);
 #if ATX_STRICT
   (void)pn_io;
   (void)radk;
@@ -462,6 +486,8 @@ template <bool BEND, bool RAD>
 __device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdouble* op, int nslice,
                                       double pn0, double irho, double radc) {
   const int nop = (int)op[0];
+  double blo, alo;
+  horner_addends<-1>(a, b, n, blo, alo);
 #if ATX_STRICT
   const double h2 = 0.0;
 #else
@@ -475,10 +501,10 @@ __device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdoub
       } else {
         if (RAD) {
           double pn = 1.0 / (1.0 + r.dp);
-          if (BEND) rad_bnd_kick<-1>(r, a, b, n, len, irho, len * irho, radc, radc * len, pn);
-          else rad_str_kick<-1>(r, a, b, n, len, radc, radc * len, pn);
+          if (BEND) rad_bnd_kick<-1>(r, a, b, n, blo, alo, len, irho, len * irho, radc, radc * len, pn);
+          else rad_str_kick<-1>(r, a, b, n, blo, alo, len, radc, radc * len, pn);
         } else {
-          if (BEND) bnd_kick<-1>(r, a, b, n, len, irho, len * irho); else str_kick<-1>(r, a, b, n, len);
+          if (BEND) bnd_kick<-1>(r, a, b, n, blo, alo, len, irho, len * irho); else str_kick<-1>(r, a, b, n, blo, alo, len);
         }
       }
     }
@@ -507,6 +533,8 @@ __device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
   } else {
     const double k1 = s[S_K1], k2 = s[S_K2];
     const double ki1 = k1 * irho, ki2 = k2 * irho;
+    double blo, alo;
+    horner_addends<N>(a, b, n, blo, alo);
 #if ATX_STRICT
     const double h2 = 0.0;
 #else
@@ -514,11 +542,11 @@ __device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
 #endif
     for (int m = 0; m < h.nslice; m++) {
       fastdrift(r, nl1, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r, a, b, n, blo, alo, k1);
       fastdrift(r, nl2, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, k2, irho, ki2); else str_kick<N>(r, a, b, n, k2);
+      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2); else str_kick<N>(r, a, b, n, blo, alo, k2);
       fastdrift(r, nl2, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, k1, irho, ki1); else str_kick<N>(r, a, b, n, k1);
+      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r, a, b, n, blo, alo, k1);
       fastdrift(r, nl1, h2);
     }
   }
@@ -548,18 +576,20 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
   } else {
     const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
     const double ki1 = k1 * irho, ki2 = k2 * irho;
+    double blo, alo;
+    horner_addends<N>(a, b, n, blo, alo);
 #if ATX_STRICT
     double pn = 0.0;  // unused: the strict steps divide as the reference does
     for (int m = 0; m < h.nslice; m++) {
       rad_drift(r, l1);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, 0.0, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, 0.0, pn);
       rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, k2, radc, 0.0, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k2, radc, 0.0, pn);
       rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, 0.0, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, 0.0, pn);
       rad_drift(r, l1);
     }
 #else
@@ -567,14 +597,14 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
     double pn = 1.0 / (1.0 + r.dp);  // one division per magnet
     for (int m = 0; m < h.nslice; m++) {
       rad_drift_pn(r, l1, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, rk1, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, rk1, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, rk1, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, rk1, pn);
       rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k2, irho, ki2, radc, rk2, pn);
-      else rad_str_kick<N>(r, a, b, n, k2, radc, rk2, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2, radc, rk2, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k2, radc, rk2, pn);
       rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, k1, irho, ki1, radc, rk1, pn);
-      else rad_str_kick<N>(r, a, b, n, k1, radc, rk1, pn);
+      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, rk1, pn);
+      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, rk1, pn);
       rad_drift_pn(r, l1, pn);
     }
 #endif
@@ -606,11 +636,11 @@ __device__ __forceinline__ void thin(R6& r, cDevElem& h, cdouble* pool) {
   entry_xform(r, h, pool);
   apertures(r, h, pool);
   switch (h.order) {
-    case 0: str_kick<0>(r, a, b, 0, 1.0); break;
-    case 1: str_kick<1>(r, a, b, 1, 1.0); break;
-    case 2: str_kick<2>(r, a, b, 2, 1.0); break;
-    case 3: str_kick<3>(r, a, b, 3, 1.0); break;
-    default: str_kick<-1>(r, a, b, h.order, 1.0); break;
+    case 0: str_kick<0>(r, a, b, 0, 0.0, 0.0, 1.0); break;
+    case 1: str_kick<1>(r, a, b, 1, b[0], a[0], 1.0); break;
+    case 2: str_kick<2>(r, a, b, 2, b[1], a[1], 1.0); break;
+    case 3: str_kick<3>(r, a, b, 3, b[2], a[2], 1.0); break;
+    default: str_kick<-1>(r, a, b, h.order, 0.0, 0.0, 1.0); break;
   }
   const double bax = s[S_BAX], bay = s[S_BAY];
   r.px += bax * r.dp;
@@ -673,7 +703,7 @@ __device__ __forceinline__ void exact_rad_drift(R6& r, double L) {
 __device__ __forceinline__ void exact_rad_kick(R6& r, cdouble* a, cdouble* b, int n, double kl,
                                                double radc) {
   double re, im;
-  field<-1>(r, a, b, n, re, im);
+  field<-1>(r, a, b, n, 0.0, 0.0, re, im);
   double pn = 1.0 / (1.0 + r.dp);
   const double xpr = r.px * pn;
   const double ypr = r.py * pn;
@@ -751,11 +781,11 @@ __device__ __noinline__ R6 exact_slices(R6 r, cdouble* a, cdouble* b, int n, int
   } else {
     for (int m = 0; m < nslice; m++) {
       exact_drift(r, l1);
-      str_kick<-1>(r, a, b, n, k1);
+      str_kick<-1>(r, a, b, n, 0.0, 0.0, k1);
       exact_drift(r, l2);
-      str_kick<-1>(r, a, b, n, k2);
+      str_kick<-1>(r, a, b, n, 0.0, 0.0, k2);
       exact_drift(r, l2);
-      str_kick<-1>(r, a, b, n, k1);
+      str_kick<-1>(r, a, b, n, 0.0, 0.0, k1);
       exact_drift(r, l1);
     }
   }
