@@ -244,6 +244,83 @@ def bench_serve(args) -> dict:
     return out
 
 
+def bench_monitors(args) -> dict:
+    """Monitor-heavy tracking: the large ring with a refpt before every
+    element and at the end (2,242 copies of every particle per turn, the
+    at.c output layout (6, N, R, T) kept in HBM), ``--particles`` particles,
+    ``--steps`` launches of one turn each. The copies are 48 B per particle
+    and refpt; the line reports their HBM bandwidth against 8 TB/s."""
+    import numpy as np
+    import torch
+
+    from at_amd import engine
+    from at_amd.flops import tracked_flops_and_passes
+    from at_amd.workloads import gaussian_beam, large_ring
+
+    dev = torch.device("cuda:0")
+    ring = large_ring()
+    nelem = len(ring)
+    n = args.particles
+    refs = np.arange(nelem + 1, dtype=np.uint32)
+    R = refs.size
+    beam = gaussian_beam(n, seed=0, scale=10.0)
+    dl = engine.DeviceLattice(ring)
+    stream = torch.cuda.current_stream()
+    r_dev = torch.from_numpy(np.ascontiguousarray(beam.T)).to(dev)
+    rout = torch.empty((R, n, 6), dtype=torch.float64, device=dev)
+    K, W = args.steps, args.warmup
+    lf = torch.zeros(n, dtype=torch.uint8, device=dev)
+    lt = torch.zeros(n, dtype=torch.int32, device=dev)
+    le = torch.zeros(n, dtype=torch.int32, device=dev)
+    lc = torch.zeros((n, 6), dtype=torch.float64, device=dev)
+
+    def launch(step):
+        dl.track_device(r_dev.data_ptr(), n, 1, refpts=refs, rout_ptr=rout.data_ptr(),
+                        turn=step, energy=ring.energy, rest_energy=ring.particle.rest_energy,
+                        losses=True, loss_ptrs=(lf.data_ptr(), lt.data_ptr(), le.data_ptr(),
+                                                lc.data_ptr()),
+                        stream=stream.cuda_stream, strict=args.strict)
+
+    for s in range(W):
+        launch(s)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(K)]
+    flops = []
+    alive_before = ~lf.cpu().numpy().astype(bool)
+    for k in range(K):
+        ev[k][0].record(stream)
+        launch(W + k)
+        ev[k][1].record(stream)
+        torch.cuda.synchronize()
+        lost = lf.cpu().numpy().astype(bool)
+        fl, _ = tracked_flops_and_passes(ring, 1, W + k, (lost & alive_before)[alive_before],
+                                         lt.cpu().numpy().view(np.uint32)[alive_before],
+                                         le.cpu().numpy().view(np.uint32)[alive_before])
+        flops.append(fl)
+        alive_before = ~lost
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    # per launch: state in and out, the refpt copies, the loss map zeroed
+    # (57 B per particle) and written for lost particles
+    nbytes = n * (48 + 48 + 57) + 48 * n * R
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    tflops = float(np.mean(flops)) / (ms * 1e-3) / 1e12
+    return {
+        "metric": "refpt-heavy tracking: HBM GB/s of the turn-by-turn copies",
+        "value": gbs, "unit": "GB/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"large_ring ({nelem} elements), {n} particles, refpts at "
+                               f"every element and the end ({R} copies per particle per "
+                               f"turn), 1 turn per launch, output (6, N, R, 1) in HBM",
+                   "bytes_per_launch": nbytes},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                     "frac": gbs / 8000.0, "traffic": None},
+        "fp64_tflops": tflops, "fp64_frac": tflops / PEAK_FP64_TFLOPS,
+    }
+
+
 def bench_adjoint(args) -> dict:
     """Differentiation: gradient of L = sum(x^2+px^2+y^2+py^2) after 64
     turns of the large ring (radiation off, cavity on) over 4,096 Gaussian
@@ -329,10 +406,12 @@ def bench_adjoint(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint"), default="track",
+    ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint", "monitors"),
+                    default="track",
                     help="track (default): the large-ring tracking line; batch: many error "
                          "realisations in one launch; serve: continuous batching of a request "
-                         "trace; adjoint: strength gradients")
+                         "trace; adjoint: strength gradients; monitors: refpts at every "
+                         "element (HBM bandwidth of the copies)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-turns", type=int, default=256)
     ap.add_argument("--no-repack", action="store_true",
@@ -381,7 +460,8 @@ def main() -> int:
     torch.cuda.set_device(dev)
     engine.set_device(0)
     if args.mode != "track":
-        fn = {"batch": bench_batch, "serve": bench_serve, "adjoint": bench_adjoint}[args.mode]
+        fn = {"batch": bench_batch, "serve": bench_serve, "adjoint": bench_adjoint,
+              "monitors": bench_monitors}[args.mode]
         print(json.dumps(fn(args)), flush=True)
         return 0
     ring = large_ring()
