@@ -178,32 +178,146 @@ class Server:
         ``realtime``; otherwise all are queued at once). Returns results in
         request order and a stats dict. ``log(result)`` is called as each
         request completes."""
+        pending = collections.deque(sorted(requests, key=lambda q: (q.arrival, q.rid)))
+
+        def fetch(tnow, block):
+            out = []
+            while pending and (not realtime or pending[0].arrival <= tnow):
+                q = pending.popleft()
+                out.append((q, q.arrival if realtime else 0.0))
+            if block and not out and pending:
+                time.sleep(max(0.0, min(pending[0].arrival - tnow, 0.05)))
+            return out, bool(pending)
+
+        results = {}
+
+        def deliver(res):
+            results[res.rid] = res
+            if log is not None:
+                log(res)
+
+        wall = self._serve(fetch, deliver)
+        res = [results[q.rid] for q in requests]
+        return res, _stats(res, wall, self.energy, self.launches)
+
+    # --- online use: requests submitted from other threads ------------------
+
+    def start(self) -> None:
+        """Serve in a background thread: ``submit`` requests while it runs,
+        ``stop`` to finish. Each request's arrival is the time of its
+        ``submit`` call."""
+        import queue
+        import threading
+
+        if getattr(self, "_thread", None) is not None:
+            raise RuntimeError("server already started")
+        self._inbox = queue.Queue()
+        self._stopping = threading.Event()
+        self._t0 = time.perf_counter()
+        self._error = None
+        device = self.device
+
+        def work():
+            import torch
+
+            torch.cuda.set_device(device)
+            try:
+                self._serve(self._fetch_online, self._deliver_online)
+            except BaseException as exc:  # noqa: BLE001 - reported through the futures
+                self._error = exc
+                while True:
+                    try:
+                        _, fut = self._inbox.get_nowait()[0:2]
+                    except queue.Empty:
+                        break
+                    fut.set_exception(exc)
+                for fut in list(self._futures.values()):
+                    if not fut.done():
+                        fut.set_exception(exc)
+
+        self._futures = {}
+        self._thread = threading.Thread(target=work, name="atx-server", daemon=True)
+        self._thread.start()
+
+    def submit(self, req: Request):
+        """Queue a request; returns a concurrent.futures.Future of its Result."""
+        from concurrent.futures import Future
+
+        if getattr(self, "_thread", None) is None or self._stopping.is_set():
+            raise RuntimeError("server is not running")
+        if self._error is not None:
+            raise RuntimeError("server stopped on an error") from self._error
+        fut = Future()
+        self._futures[req.rid] = fut
+        self._inbox.put((req, fut, time.perf_counter() - self._t0))
+        return fut
+
+    def stop(self, timeout: float | None = None) -> None:
+        """Serve what was submitted, then end the background thread."""
+        if getattr(self, "_thread", None) is None:
+            return
+        self._stopping.set()
+        self._thread.join(timeout)
+        self._thread = None
+        if self._error is not None:
+            raise RuntimeError("server stopped on an error") from self._error
+
+    def _fetch_online(self, tnow, block):
+        import queue
+
+        out = []
+        try:
+            if block:
+                item = self._inbox.get(timeout=0.05)
+                out.append(item)
+            while True:
+                out.append(self._inbox.get_nowait())
+        except queue.Empty:
+            pass
+        # arrival = submit time, on the server's clock (the loop's t0 is later)
+        arrivals = [(req, max(0.0, t - self._loop_t0_offset)) for req, _, t in out]
+        more = not (self._stopping.is_set() and self._inbox.empty())
+        return arrivals, more
+
+    def _deliver_online(self, res):
+        fut = self._futures.pop(res.rid, None)
+        if fut is not None:
+            fut.set_result(res)
+
+    def _serve(self, fetch, deliver) -> float:
+        """The scheduling loop. fetch(tnow, block) -> (list of (request,
+        arrival), more arrivals possible); block=True when the server has
+        nothing to do until a request arrives. deliver(result) receives each
+        completed request. Returns the wall time."""
         import torch
 
-        pending = collections.deque(sorted(requests, key=lambda q: (q.arrival, q.rid)))
         queue = collections.deque()
         active: list[_Slot] = []
         finishing = collections.deque()   # (slot, finish event, copy-done event)
         inflight = collections.deque()    # launch-end events
-        results = {}
+        arrival = {}
         used = 0
         torch.cuda.synchronize()
         t0_ev = torch.cuda.Event(enable_timing=True)
         t0_ev.record(self.stream)
         t0 = time.perf_counter()
+        self._loop_t0_offset = t0 - getattr(self, "_t0", t0)
 
         def now():
             return time.perf_counter() - t0
 
-        while pending or queue or active or finishing:
+        more = True
+        while more or queue or active or finishing:
             tnow = now()
-            while pending and (not realtime or pending[0].arrival <= tnow):
-                queue.append(pending.popleft())
+            new, more = fetch(tnow, False)
+            for q, t in new:
+                arrival[q.rid] = t
+                queue.append(q)
             while (queue and len(active) < self.max_active
                    and used + self.need(queue[0]) <= self.budget):
                 q = queue.popleft()
                 used += self.need(q)
-                active.append(self._admit(q, max(tnow, q.arrival if realtime else 0.0)))
+                active.append(self._admit(q, max(tnow, arrival[q.rid])))
             if queue and not active and used == 0 and not finishing:
                 raise ValueError(f"request {queue[0].rid} needs {self.need(queue[0])} bytes, "
                                  f"over the HBM budget of {self.budget}")
@@ -227,31 +341,30 @@ class Server:
                 s, _, cp = finishing.popleft()
                 done = t0_ev.elapsed_time(cp) / 1000.0  # results on the host
                 h = s.host
-                results[s.req.rid] = Result(
+                used -= self.need(s.req)
+                deliver(Result(
                     s.req.rid, h["r"].numpy().T.copy(),
                     {"islost": h["flag"].numpy().astype(bool),
                      "turn": h["turn"].numpy().view(np.uint32).copy(),
                      "elem": h["elem"].numpy().view(np.uint32).copy(),
                      "coord": h["coord"].numpy().T.copy()},
-                    s.req.arrival if realtime else 0.0, s.admitted, done)
-                used -= self.need(s.req)
-                if log is not None:
-                    log(results[s.req.rid])
+                    arrival.pop(s.req.rid), s.admitted, done))
             while inflight and inflight[0].query():
                 inflight.popleft()
             if launched:
                 continue
             # nothing launched: wait for the device or for the next arrival
-            if inflight and (len(inflight) >= self.depth or not (queue or pending)):
+            if inflight and (len(inflight) >= self.depth or not (queue or more)):
                 inflight.popleft().synchronize()
             elif finishing:
                 finishing[0][2].synchronize()
-            elif pending and not queue:
-                time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+            elif not queue and more:
+                new, more = fetch(now(), True)
+                for q, t in new:
+                    arrival[q.rid] = t
+                    queue.append(q)
         torch.cuda.synchronize()
-        wall = now()
-        res = [results[q.rid] for q in requests]
-        return res, _stats(res, wall, self.energy, self.launches)
+        return now()
 
 
 def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realtime: bool = True,

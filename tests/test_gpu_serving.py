@@ -136,3 +136,41 @@ def test_budget_too_small_is_an_error():
                  hbm_budget=1000)
     with pytest.raises(ValueError, match="HBM budget"):
         srv.run([Request(0, 0, gaussian_beam(100), 2)], realtime=False)
+
+
+def test_online_submit_from_threads_equals_alone():
+    """start/submit/stop: requests submitted while the server runs (from two
+    client threads, at different times) give the same results as alone."""
+    import threading
+    import time
+
+    lats = _instances()
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dls = [engine.DeviceLattice(l) for l in lats]
+    trace = _trace(8, seed=6)
+    srv = Server(dls, energy=energy, rest_energy=rest, turn_block=8, max_active=3,
+                 hbm_budget=5000 * 105)
+    srv.start()
+    futs = {}
+
+    def client(reqs):
+        for q in reqs:
+            futs[q.rid] = srv.submit(q)
+            time.sleep(0.01)
+
+    threads = [threading.Thread(target=client, args=(trace[k::2],)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    results = {rid: f.result(timeout=120) for rid, f in futs.items()}
+    srv.stop()
+    with pytest.raises(RuntimeError):
+        srv.submit(trace[0])
+    for q in trace:
+        res = results[q.rid]
+        r_alone, lm = _alone(dls[q.instance], q, energy, rest)
+        np.testing.assert_array_equal(res.particles, r_alone)
+        np.testing.assert_array_equal(res.lost["turn"], lm["turn"])
+        np.testing.assert_array_equal(res.lost["elem"], lm["elem"])
+        assert res.done >= res.arrival
