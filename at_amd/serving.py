@@ -46,6 +46,7 @@ class Request:
     nturns: int
     arrival: float = 0.0      # seconds after the start of the trace
     kind: str = "track"       # "track" or "da": both return coordinates and loss map
+    deadline: float | None = None  # seconds after arrival; then the request is ended
 
     @property
     def npart(self) -> int:
@@ -60,6 +61,8 @@ class Result:
     arrival: float
     admitted: float
     done: float
+    status: str = "done"      # "done", "timeout" (deadline passed) or "cancelled"
+    turns_done: int = -1      # turns tracked; the request's nturns when done
 
     @property
     def latency(self) -> float:
@@ -76,6 +79,7 @@ class _Slot:
     coord: object
     done_turns: int = 0
     admitted: float = 0.0
+    status: str = "done"
     host: dict = field(default_factory=dict)
 
 
@@ -119,6 +123,7 @@ class Server:
         for dl in self.dls:  # compile and upload every instance now, not on first use
             dl.check(self.energy, self.rest)
         self.launches = 0
+        self._cancel = set()
 
     def need(self, req: Request) -> int:
         return req.npart * BYTES_PER_PARTICLE
@@ -252,6 +257,12 @@ class Server:
         self._inbox.put((req, fut, time.perf_counter() - self._t0))
         return fut
 
+    def cancel(self, rid: int) -> None:
+        """End a request early: a queued one is dropped, an active one stops
+        after the turns already launched and returns its state at that point
+        (status "cancelled"). Safe from any thread."""
+        self._cancel.add(int(rid))
+
     def stop(self, timeout: float | None = None) -> None:
         """Serve what was submitted, then end the background thread."""
         if getattr(self, "_thread", None) is None:
@@ -261,6 +272,13 @@ class Server:
         self._thread = None
         if self._error is not None:
             raise RuntimeError("server stopped on an error") from self._error
+
+    def _ended(self, req: Request, arrival: float, tnow: float) -> str:
+        if req.rid in self._cancel:
+            return "cancelled"
+        if req.deadline is not None and tnow - arrival >= req.deadline:
+            return "timeout"
+        return ""
 
     def _fetch_online(self, tnow, block):
         import queue
@@ -313,6 +331,37 @@ class Server:
             for q, t in new:
                 arrival[q.rid] = t
                 queue.append(q)
+            # deadlines and cancellations: queued requests are dropped, active
+            # ones end after the turns already launched
+            if self._cancel or any(q.deadline is not None for q in queue) or any(
+                    sl.req.deadline is not None for sl in active):
+                ended = []
+                for q in list(queue):
+                    why = self._ended(q, arrival[q.rid], tnow)
+                    if why:
+                        queue.remove(q)
+                        ended.append((q, why))
+                for q, why in ended:
+                    self._cancel.discard(q.rid)
+                    deliver(Result(q.rid, np.asarray(q.particles).copy(), _empty_loss(q.npart),
+                                   arrival.pop(q.rid), tnow, tnow, why, 0))
+                still = []
+                for sl in active:
+                    why = self._ended(sl.req, arrival[sl.req.rid], tnow)
+                    if why and sl.done_turns > 0:
+                        sl.status = why
+                        fin, cp = self._copy_back(sl)
+                        finishing.append((sl, fin, cp))
+                    elif why:
+                        # admitted but not launched yet: its buffers are free to go
+                        used -= self.need(sl.req)
+                        self._cancel.discard(sl.req.rid)
+                        deliver(Result(sl.req.rid, np.asarray(sl.req.particles).copy(),
+                                       _empty_loss(sl.req.npart), arrival.pop(sl.req.rid),
+                                       sl.admitted, tnow, why, 0))
+                    else:
+                        still.append(sl)
+                active = still
             while (queue and len(active) < self.max_active
                    and used + self.need(queue[0]) <= self.budget):
                 q = queue.popleft()
@@ -348,7 +397,8 @@ class Server:
                      "turn": h["turn"].numpy().view(np.uint32).copy(),
                      "elem": h["elem"].numpy().view(np.uint32).copy(),
                      "coord": h["coord"].numpy().T.copy()},
-                    arrival.pop(s.req.rid), s.admitted, done))
+                    arrival.pop(s.req.rid), s.admitted, done, s.status, s.done_turns))
+                self._cancel.discard(s.req.rid)
             while inflight and inflight[0].query():
                 inflight.popleft()
             if launched:
@@ -423,6 +473,11 @@ def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realti
     wall = time.perf_counter() - t0
     res = [results[q.rid] for q in requests]
     return res, _stats(res, wall, energy, len(order))
+
+
+def _empty_loss(n: int) -> dict:
+    return {"islost": np.zeros(n, bool), "turn": np.zeros(n, np.uint32),
+            "elem": np.zeros(n, np.uint32), "coord": np.zeros((6, n))}
 
 
 def _stats(results, wall: float, energy: float, launches: int) -> dict:

@@ -174,3 +174,39 @@ def test_online_submit_from_threads_equals_alone():
         np.testing.assert_array_equal(res.lost["turn"], lm["turn"])
         np.testing.assert_array_equal(res.lost["elem"], lm["elem"])
         assert res.done >= res.arrival
+
+
+def test_deadline_and_cancel_end_requests_early():
+    """A request whose deadline passes stops after the turns already
+    launched and returns its state there, equal to tracking alone for that
+    many turns; a queued request past its deadline is dropped untouched; a
+    cancelled request of a live server ends the same way."""
+    import time
+
+    lats = _instances(1)
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    dl = engine.DeviceLattice(lats[0])
+    long_q = Request(0, 0, gaussian_beam(20000, seed=3, scale=5.0), 100000, deadline=0.5)
+    late_q = Request(1, 0, gaussian_beam(100, seed=4), 10, arrival=0.0, deadline=0.0)
+    srv = Server([dl], energy=energy, rest_energy=rest, turn_block=16, max_active=1)
+    (r0, r1), _ = srv.run([long_q, late_q], realtime=False)
+    assert r0.status == "timeout" and 0 < r0.turns_done < long_q.nturns
+    assert r0.turns_done % 16 == 0
+    r = np.asfortranarray(long_q.particles.copy())
+    _, lm = dl.track(r, r0.turns_done, None, energy=energy, rest_energy=rest, losses=True)
+    np.testing.assert_array_equal(r0.particles, r)
+    np.testing.assert_array_equal(r0.lost["turn"], lm["turn"])
+    assert r1.status == "timeout" and r1.turns_done == 0
+    np.testing.assert_array_equal(r1.particles, late_q.particles)
+
+    srv2 = Server([dl], energy=energy, rest_energy=rest, turn_block=16)
+    srv2.start()
+    fut = srv2.submit(Request(7, 0, gaussian_beam(20000, seed=5, scale=5.0), 100000))
+    time.sleep(0.5)
+    srv2.cancel(7)
+    res = fut.result(timeout=60)
+    srv2.stop()
+    assert res.status == "cancelled" and 0 < res.turns_done < 100000
+    r = np.asfortranarray(gaussian_beam(20000, seed=5, scale=5.0))
+    dl.track(r, res.turns_done, None, energy=energy, rest_energy=rest)
+    np.testing.assert_array_equal(res.particles, r)
