@@ -24,6 +24,7 @@ from .acceptance import (  # noqa: E402,F401
     get_momentum_acceptance, get_vertical_acceptance,
 )
 from .envelope import get_tunes_damp, ohmi_envelope  # noqa: E402,F401
+from .particles import beam, emittances_from_beam, sigma_matrix  # noqa: E402,F401
 from .physics import (  # noqa: E402,F401
     DConstant, ELossMethod, find_elem_m66, find_m44, find_m66, find_orbit, find_orbit4,
     find_orbit6, find_sync_orbit, get_energy_loss, get_timelag_fromU0,

@@ -1,4 +1,4 @@
-"""GPU: the equilibrium envelope on the engine (diffusion matrices from
+"""GPU: the equilibrium envelope and the ring's particle distribution on the engine (diffusion matrices from
 atx_diffusion_matrices, orbits and element matrices from the HIP tracking)
 against the MATLAB known answers of tests/test_envelope.py, and the
 cumulative diffusion against the same computation with the reference's
@@ -11,6 +11,7 @@ from at_amd import envelope
 
 import oracle_hooks
 from test_envelope import hmba6, test_ohmi_envelope_known_answers  # noqa: F401
+from test_particles import test_ring_4d_needs_emittances, test_ring_6d_equilibrium  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
