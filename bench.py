@@ -428,7 +428,7 @@ def main() -> int:
     ap.add_argument("--particles", type=int, default=262144)
     ap.add_argument("--strict", action="store_true", help="reference operation order build")
     ap.add_argument("--cpu-particles", type=int, default=32768)
-    ap.add_argument("--cpu-turns", type=int, default=2)
+    ap.add_argument("--cpu-turns", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 times): the "
