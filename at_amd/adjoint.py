@@ -12,13 +12,22 @@ backward is the hand-written adjoint kernel (atx_adjoint):
   (atx_track_batch, one instance per block with its own turn number), and
   the state at the start of each turn is kept; the blocks' arithmetic is
   that of the forward pass, so the states are bit-identical to it;
-* it then walks the blocks from the last, running the adjoint kernel over
-  each, which reconstructs every element's input by exact inversion and
-  accumulates dL/dPolynomB[k] of the selected elements.
+* it then runs the adjoint kernel over the blocks, which reconstructs every
+  element's input by exact inversion and accumulates dL/dPolynomB[k] of the
+  selected elements. Sequentially, block after block from the last; or,
+  when the beam is too small to fill the device, in parallel: the adjoint
+  is linear in lambda, so one launch takes unit seeds e_0..e_5 through every
+  block but the first (rows of each block's transposed Jacobian, one 6x6
+  matrix per particle) together with the last block's own sweep; lambda at
+  every block boundary then follows by 6x6 products, and a second launch
+  sweeps the remaining blocks together, accumulating the gradient. That is
+  two launches of 8 turns instead of nblocks in sequence, for 7 times the
+  lane work.
 
 Memory: checkpoints (nturns/checkpoint + 1) x N x 48 B plus the turn-start
 states of all blocks, (nturns + nblocks) x N x 48 B (12.6 MB for 4,096
-particles and 64 turns).
+particles and 64 turns); the parallel sweep adds (nblocks - 1) x N x 288 B
+of block matrices.
 """
 
 from __future__ import annotations
@@ -31,15 +40,19 @@ from . import engine
 class Problem:
     """A lattice, its differentiation parameters and the tracking setup.
 
-    params: sequence of (element index, PolynomB index)."""
+    params: sequence of (element index, PolynomB index). parallel: sweep
+    the blocks in parallel (True), in sequence (False) or as a cost model of
+    the device's occupancy prefers (None)."""
 
-    def __init__(self, lattice, params, *, energy=None, rest_energy=None, checkpoint: int = 8):
+    def __init__(self, lattice, params, *, energy=None, rest_energy=None, checkpoint: int = 8,
+                 parallel: bool | None = None):
         self.lattice = lattice
         self.params = [(int(e), int(k)) for e, k in params]
         self.energy = float(lattice.energy if energy is None else energy)
         part = getattr(lattice, "particle", None)
         self.rest = float(getattr(part, "rest_energy", 0.0) if rest_energy is None else rest_energy)
         self.checkpoint = int(checkpoint)
+        self.parallel = parallel
         self.dl = engine.DeviceLattice(lattice)
         n = len(lattice)
         self.slots = np.full(n, -1, np.int32)
@@ -62,6 +75,75 @@ class Problem:
             pb[k] = v
         for e, pb in touched.items():
             self.dl.set_array_at(e, "PolynomB", pb)
+
+
+def _sweep_cost(lanes: int, simds: int) -> float:
+    """Time of one sweep over `lanes` particles, in units of a sweep with at
+    most one wave per SIMD: below that the dependent FP64 chain of each
+    lane sets the time; above it each further wave per SIMD adds about 0.64
+    of a wave's time (the tracking kernel takes 2.57 times as long with 4
+    waves per SIMD as with 1)."""
+    return max(1.0, 0.64 * lanes / (64.0 * simds))
+
+
+def use_parallel_sweep(npart: int, nblocks: int, simds: int) -> bool:
+    """The parallel sweep (about 7 times the lane work, in two launches)
+    against nblocks sequential launches."""
+    if nblocks < 2:
+        return False
+    seq = nblocks * _sweep_cost(npart, simds)
+    par = (_sweep_cost((6 * (nblocks - 1) + 1) * npart, simds)
+           + _sweep_cost((nblocks - 1) * npart, simds))
+    return par < 0.8 * seq
+
+
+def _device_simds(dev) -> int:
+    import torch
+
+    return 4 * torch.cuda.get_device_properties(dev).multi_processor_count
+
+
+def _sweep_sequential(problem, blocks, states, lam, grad, n, stream):
+    npar = len(problem.params)
+    for k in reversed(range(len(blocks))):
+        t0, nt, _ = blocks[k]
+        problem.dl.adjoint(states[k].data_ptr(), nt, t0, n, problem.energy, problem.rest,
+                           lam.data_ptr(), problem.slots, problem.coeffs, grad.data_ptr(), npar,
+                           stream=stream)
+
+
+def _sweep_parallel(problem, blocks, states, lam, grad, n, stream):
+    """Blocks in parallel: unit seeds through blocks 1..B-1 (with block
+    B-1's own sweep in the same launch), lambda at the block boundaries by
+    6x6 products, then blocks 0..B-2 in one launch."""
+    import torch
+
+    nb = len(blocks)
+    npar = len(problem.params)
+    # seeds[k - 1, i]: adjoint of the end of block k seeded with e_i; after
+    # the launch, row i of block k's transposed Jacobian for every particle
+    seeds = torch.zeros((nb - 1, 6, n, 6), dtype=torch.float64, device=lam.device)
+    for i in range(6):
+        seeds[:, i, :, i] = 1.0
+    items = []
+    for k in range(1, nb):
+        t0, nt, _ = blocks[k]
+        for i in range(6):
+            items.append((states[k].data_ptr(), seeds[k - 1, i].data_ptr(), nt, t0, n, False))
+    t0, nt, _ = blocks[-1]
+    items.append((states[-1].data_ptr(), lam.data_ptr(), nt, t0, n, True))
+    problem.dl.adjoint_batch(items, problem.energy, problem.rest, problem.slots, problem.coeffs,
+                             grad.data_ptr(), npar, stream=stream)
+    # ends[k]: lambda at the end of block k (= at the start of block k + 1)
+    ends = {nb - 2: lam}
+    for k in range(nb - 2, 0, -1):
+        # rows[p, i]: lambda at the start of block k for lambda = e_i at its end
+        rows = seeds[k - 1].permute(1, 0, 2)
+        ends[k - 1] = (ends[k][:, :, None] * rows).sum(dim=1).contiguous()
+    items = [(states[k].data_ptr(), ends[k].data_ptr(), blocks[k][1], blocks[k][0], n, True)
+             for k in range(nb - 1)]
+    problem.dl.adjoint_batch(items, problem.energy, problem.rest, problem.slots, problem.coeffs,
+                             grad.data_ptr(), npar, stream=stream)
 
 
 def _loss_and_seed(r_end):
@@ -130,11 +212,13 @@ class TrackingLoss:
                                        rest_energy=problem.rest, losses=False, stream=stream)
                 for k, (_, nt, _) in enumerate(blocks):
                     states[k][nt].copy_(cur[k])
-                for k in reversed(range(len(blocks))):
-                    t0, nt, _ = blocks[k]
-                    problem.dl.adjoint(states[k].data_ptr(), nt, t0, n, problem.energy,
-                                       problem.rest, lam.data_ptr(), problem.slots,
-                                       problem.coeffs, grad.data_ptr(), npar, stream=stream)
+                par = problem.parallel
+                if par is None:
+                    par = use_parallel_sweep(n, len(blocks), _device_simds(particles.device))
+                if par and len(blocks) > 1:
+                    _sweep_parallel(problem, blocks, states, lam, grad, n, stream)
+                else:
+                    _sweep_sequential(problem, blocks, states, lam, grad, n, stream)
                 return grad.to(device=ctx.theta_device, dtype=torch.float64)
 
         return _F.apply(strengths)

@@ -23,6 +23,7 @@ namespace adj {
 typedef __attribute__((address_space(4))) const double cdouble;
 typedef __attribute__((address_space(4))) const DevElem cDevElem;
 typedef __attribute__((address_space(4))) const AdjElem cAdjElem;
+typedef __attribute__((address_space(4))) const AdjItem cAdjItem;
 
 struct V6 {
   double x, px, y, py, dp, ct;
@@ -167,7 +168,7 @@ __device__ __forceinline__ void mpole_rev(V6& r, V6& l, cDevElem& h, cAdjElem& a
   cdouble* a = pool + h.poly;
   cdouble* b = a + h.order + 1;
   const int n = h.order;
-  const int jg = ae.slot >= 0 ? ae.coeff : -1;
+  const int jg = (ae.slot >= 0 && grad) ? ae.coeff : -1;
   double g = 0.0;
   const double irho = s[S_IRHO];
   exit_rev(r, l, h, ae, pool);
@@ -184,7 +185,7 @@ __device__ __forceinline__ void mpole_rev(V6& r, V6& l, cDevElem& h, cAdjElem& a
   }
   if (BEND) edge_rev(r, l, h.edge & 0xff, s[S_FX_IN], s[S_FY_IN]);
   entry_rev(r, l, h, ae, pool);
-  if (ae.slot >= 0) grad_flush(g, ae.slot, grad);
+  if (jg >= 0) grad_flush(g, ae.slot, grad);
 }
 
 template <bool BEND>
@@ -203,7 +204,7 @@ __device__ __forceinline__ void thin_rev(V6& r, V6& l, cDevElem& h, cAdjElem& ae
   cdouble* s = pool + h.sc;
   cdouble* a = pool + h.poly;
   cdouble* b = a + h.order + 1;
-  const int jg = ae.slot >= 0 ? ae.coeff : -1;
+  const int jg = (ae.slot >= 0 && grad) ? ae.coeff : -1;
   double g = 0.0;
   exit_rev(r, l, h, ae, pool);
   const double bax = s[S_BAX], bay = s[S_BAY];
@@ -216,7 +217,7 @@ __device__ __forceinline__ void thin_rev(V6& r, V6& l, cDevElem& h, cAdjElem& ae
   l.dp += bax * l.px - bay * l.py;
   kick_rev<false, -1>(r, l, a, b, h.order, 1.0, 0.0, jg, g);
   entry_rev(r, l, h, ae, pool);
-  if (ae.slot >= 0) grad_flush(g, ae.slot, grad);
+  if (jg >= 0) grad_flush(g, ae.slot, grad);
 }
 
 // full drift with its own 1/(1+dp): same Jacobian as drift_rev
@@ -280,31 +281,39 @@ __device__ __forceinline__ void element_rev(V6& r, V6& l, cDevElem& h, cAdjElem&
   }
 }
 
+// grid: x over the particles of the largest item, y over the items; lanes
+// beyond an item's particle count idle with zero adjoint (a workgroup with
+// none of its particles returns at once: the kernel has no barrier)
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs args) {
+  cAdjItem& it = ((cAdjItem*)args.items)[blockIdx.y];
+  const int npart = it.npart;
+  if ((int)(blockIdx.x * blockDim.x) >= npart) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = p < args.npart;
-  const int64_t np6 = 6 * (int64_t)args.npart;
+  const bool active = p < npart;
+  const int64_t np6 = 6 * (int64_t)npart;
+  const double* states = it.states;
+  double* grad = it.grad;
   cDevElem* elems = (cDevElem*)args.elems;
   cAdjElem* aes = (cAdjElem*)args.adj;
   cdouble* pool = (cdouble*)args.pool;
   // inactive lanes (beyond N, or particles lost by the end of the block)
   // run with zero adjoint so that every lane reaches the wave reductions
-  const double* send = args.states + (int64_t)args.nturns * np6 + 6 * (int64_t)(active ? p : 0);
+  const double* send = states + (int64_t)it.nturns * np6 + 6 * (int64_t)(active ? p : 0);
   bool live = active && __builtin_isfinite(send[0]);
   V6 l = {0, 0, 0, 0, 0, 0};
   if (live) {
-    const double* lp = args.lam + 6 * (int64_t)p;
+    const double* lp = it.lam + 6 * (int64_t)p;
     l = V6{lp[0], lp[1], lp[2], lp[3], lp[4], lp[5]};
   }
-  for (int t = args.nturns - 1; t >= 0; t--) {
-    const double* sp = args.states + (int64_t)(t + 1) * np6 + 6 * (int64_t)(active ? p : 0);
+  for (int t = it.nturns - 1; t >= 0; t--) {
+    const double* sp = states + (int64_t)(t + 1) * np6 + 6 * (int64_t)(active ? p : 0);
     V6 r = {0, 0, 0, 0, 0, 0};
     if (live) r = V6{sp[0], sp[1], sp[2], sp[3], sp[4], sp[5]};
-    const int nturn = args.turn0 + t;
-    for (int e = args.nelem - 1; e >= 0; e--) element_rev(r, l, elems[e], aes[e], pool, args.grad, nturn);
+    const int nturn = it.turn0 + t;
+    for (int e = args.nelem - 1; e >= 0; e--) element_rev(r, l, elems[e], aes[e], pool, grad, nturn);
   }
   if (active) {
-    double* lp = args.lam + 6 * (int64_t)p;
+    double* lp = it.lam + 6 * (int64_t)p;
     lp[0] = l.x;
     lp[1] = l.px;
     lp[2] = l.y;
@@ -317,10 +326,14 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs args) {
 }  // namespace adj
 }  // namespace atx
 
-extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs* args, hipStream_t stream) {
+// max_npart: the largest particle count of the items (host-side copy of the
+// device item table); nitems <= 65535 (grid y limit)
+extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs* args, int32_t max_npart,
+                                         hipStream_t stream) {
   const int block = 256;
-  const int nblocks = (args->npart + block - 1) / block;
-  if (nblocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(atx::adj::adjoint_kernel, dim3(nblocks), dim3(block), 0, stream, *args);
+  const int nblocks = (max_npart + block - 1) / block;
+  if (nblocks == 0 || args->nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::adj::adjoint_kernel, dim3(nblocks, args->nitems), dim3(block), 0, stream,
+                     *args);
   return hipGetLastError();
 }

@@ -162,17 +162,24 @@ struct DiffArgs {
   int32_t pad;
 };
 
+// One block of turns of the adjoint sweep (gridDim.y indexes the items).
+struct AdjItem {
+  const double* states;  // (nturns+1) blocks of (6, N): turn-start states, then the end state
+  double* lam;           // (6, N) adjoint of the end state in, of the start state out
+  double* grad;          // accumulated dL/dPolynomB[coeff] per slot; NULL: not accumulated
+  int32_t npart;
+  int32_t nturns;
+  int32_t turn0;         // turn counter of the first turn of the block
+  int32_t pad;
+};
+
 struct AdjArgs {
   const DevElem* elems;
   const double* pool;
   const AdjElem* adj;
-  const double* states;  // (nturns+1) blocks of (6, N): turn-start states, then the end state
-  double* lam;           // (6, N) adjoint of the end state in, of the start state out
-  double* grad;          // accumulated dL/dPolynomB[coeff] per slot
+  const AdjItem* items;
   int32_t nelem;
-  int32_t npart;
-  int32_t nturns;
-  int32_t turn0;         // turn counter of the first turn of the block
+  int32_t nitems;
 };
 
 }  // namespace atx

@@ -21,7 +21,7 @@ extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, in
 extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
 extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
-extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, hipStream_t);
+extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, int32_t, hipStream_t);
 
 namespace atx {
 
@@ -1188,13 +1188,11 @@ int atx_element_set_array_at(atx_lattice* lat, int32_t index, const char* name, 
   return ATX_OK;
 }
 
-int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t turn, uint32_t np,
-                double energy, double rest_energy, double* lam, const int32_t* grad_slot,
-                const int32_t* grad_coeff, double* grad, int32_t num_params, void* stream_) {
-  g_err.clear();
-  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
-  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
-  if (np > 0 && (!states || !lam)) return fail(ATX_E_VALUE, "states/lam are NULL");
+// Shared by atx_adjoint and atx_adjoint_batch: per-element adjoint table,
+// one upload of that table and the item table, one launch.
+static int adjoint_launch(atx_lattice* lat, const std::vector<AdjItem>& items, double energy,
+                          double rest_energy, const int32_t* grad_slot, const int32_t* grad_coeff,
+                          bool have_grad, int32_t num_params, void* stream_) {
   std::lock_guard<std::mutex> g(lat->mu);
   const int nelem = (int)lat->elems.size();
   int rc = ensure_device_lattice(lat, energy, rest_energy, false);
@@ -1210,7 +1208,7 @@ int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t 
       const int k = lat->h_kind[e];
       if (k != K_STR && k != K_BND && k != K_THIN)
         return fail(ATX_E_VALUE, "element #" + std::to_string(e) + " has no PolynomB to differentiate");
-      if (slot >= num_params || !grad) return fail(ATX_E_VALUE, "gradient slot out of range");
+      if (slot >= num_params || !have_grad) return fail(ATX_E_VALUE, "gradient slot out of range");
       if (coeff < 0 || coeff > lat->h_order[e])
         return fail(ATX_E_VALUE, "element #" + std::to_string(e) + ": PolynomB index above MaxOrder");
     }
@@ -1219,31 +1217,89 @@ int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t 
     ae[e].r1inv = lat->h_r1inv[e];
     ae[e].r2inv = lat->h_r2inv[e];
   }
-  if (np == 0 || nturns == 0) return ATX_OK;
+  std::vector<AdjItem> run;
+  int32_t max_np = 0;
+  for (const AdjItem& it : items)
+    if (it.npart > 0 && it.nturns > 0) {
+      run.push_back(it);
+      max_np = std::max(max_np, it.npart);
+    }
+  if (run.empty()) return ATX_OK;
+  // one buffer: the element table, then the items (both 8-byte aligned)
+  const size_t ae_bytes = ae.size() * sizeof(AdjElem);
+  const size_t ae_pad = (ae_bytes + 255) & ~(size_t)255;
+  std::vector<unsigned char> host(ae_pad + run.size() * sizeof(AdjItem));
+  std::memcpy(host.data(), ae.data(), ae_bytes);
+  std::memcpy(host.data() + ae_pad, run.data(), run.size() * sizeof(AdjItem));
   hipStream_t stream = (hipStream_t)stream_;
-  AdjElem* d_ae = nullptr;
-  HIPCHK(hipMallocAsync((void**)&d_ae, ae.size() * sizeof(AdjElem), stream));
-  hipError_t ce = hipMemcpyAsync(d_ae, ae.data(), ae.size() * sizeof(AdjElem), hipMemcpyHostToDevice, stream);
+  unsigned char* d_buf = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d_buf, host.size(), stream));
+  hipError_t ce = hipMemcpyAsync(d_buf, host.data(), host.size(), hipMemcpyHostToDevice, stream);
   hipError_t le = hipSuccess;
   if (ce == hipSuccess) {
     AdjArgs a;
     a.elems = lat->d_elems;
     a.pool = lat->d_pool;
-    a.adj = d_ae;
-    a.states = states;
-    a.lam = lam;
-    a.grad = grad;
+    a.adj = (const AdjElem*)d_buf;
+    a.items = (const AdjItem*)(d_buf + ae_pad);
     a.nelem = nelem;
-    a.npart = (int32_t)np;
-    a.nturns = nturns;
-    a.turn0 = turn;
-    le = atx_launch_adjoint(&a, stream);
+    a.nitems = (int32_t)run.size();
+    le = atx_launch_adjoint(&a, max_np, stream);
   }
-  hipError_t fe = hipFreeAsync(d_ae, stream);
+  hipError_t fe = hipFreeAsync(d_buf, stream);
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("adjoint launch: ") + hipGetErrorString(le));
   if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
   return ATX_OK;
+}
+
+int atx_adjoint(atx_lattice* lat, const double* states, int32_t nturns, int32_t turn, uint32_t np,
+                double energy, double rest_energy, double* lam, const int32_t* grad_slot,
+                const int32_t* grad_coeff, double* grad, int32_t num_params, void* stream_) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  if (np > 0 && (!states || !lam)) return fail(ATX_E_VALUE, "states/lam are NULL");
+  if (np > (uint32_t)INT32_MAX) return fail(ATX_E_VALUE, "num_particles too large");
+  AdjItem it{};
+  it.states = states;
+  it.lam = lam;
+  it.grad = grad;
+  it.npart = (int32_t)np;
+  it.nturns = nturns;
+  it.turn0 = turn;
+  return adjoint_launch(lat, {it}, energy, rest_energy, grad_slot, grad_coeff, grad != nullptr,
+                        num_params, stream_);
+}
+
+int atx_adjoint_batch(atx_lattice* lat, const atx_adjoint_item* items, int32_t nitems, double energy,
+                      double rest_energy, const int32_t* grad_slot, const int32_t* grad_coeff,
+                      double* grad, int32_t num_params, void* stream_) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  if (nitems < 0 || (nitems > 0 && !items)) return fail(ATX_E_VALUE, "bad item list");
+  if (nitems > 65535) return fail(ATX_E_VALUE, "more than 65535 items");
+  std::vector<AdjItem> v;
+  v.reserve((size_t)nitems);
+  for (int32_t i = 0; i < nitems; i++) {
+    const atx_adjoint_item& s = items[i];
+    if (s.nturns < 0) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": nturns must be >= 0");
+    if (s.num_particles > (uint32_t)INT32_MAX)
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": num_particles too large");
+    if (s.num_particles > 0 && (!s.states || !s.lam))
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": states/lam are NULL");
+    if (s.accumulate && !grad) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": grad is NULL");
+    AdjItem it{};
+    it.states = s.states;
+    it.lam = s.lam;
+    it.grad = s.accumulate ? grad : nullptr;
+    it.npart = (int32_t)s.num_particles;
+    it.nturns = s.nturns;
+    it.turn0 = s.turn;
+    v.push_back(it);
+  }
+  return adjoint_launch(lat, v, energy, rest_energy, grad_slot, grad_coeff, grad != nullptr, num_params,
+                        stream_);
 }
 
 int atx_diffusion_matrices(atx_lattice* lat, const double* orbits, double* bdiff, double energy,

@@ -39,7 +39,7 @@ EXPORTS = (
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
     "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
-    "atx_diffusion_matrices",
+    "atx_diffusion_matrices", "atx_adjoint_batch",
 )
 
 
@@ -50,6 +50,13 @@ class BatchItem(C.Structure):
                 ("pad", C.c_uint32), ("lost_flag", C.c_void_p), ("lost_turn", C.c_void_p),
                 ("lost_elem", C.c_void_p), ("lost_coord", C.c_void_p), ("nturns", C.c_int32),
                 ("turn", C.c_int32)]
+
+class AdjointItem(C.Structure):
+    """struct atx_adjoint_item of include/atx.h."""
+
+    _fields_ = [("states", C.c_void_p), ("lam", C.c_void_p), ("nturns", C.c_int32),
+                ("turn", C.c_int32), ("num_particles", C.c_uint32), ("accumulate", C.c_int32)]
+
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -111,6 +118,10 @@ def load_library(path: str | None = None) -> C.CDLL:
                                     C.c_double, C.c_double, C.c_void_p,
                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_void_p,
                                     C.c_int32, C.c_void_p]
+        lib.atx_adjoint_batch.argtypes = [C.c_void_p, C.POINTER(AdjointItem), C.c_int32,
+                                          C.c_double, C.c_double, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.c_void_p, C.c_int32,
+                                          C.c_void_p]
         lib.atx_element_set_array_at.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, _dp,
                                                  C.c_int64]
         lib.atx_diffusion_matrices.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
@@ -223,6 +234,25 @@ class DeviceLattice:
                                     float(energy), float(rest_energy), lam_ptr,
                                     slots.ctypes.data_as(ip), coeffs.ctypes.data_as(ip),
                                     grad_ptr, int(nparams), stream), self.lib)
+
+    def adjoint_batch(self, items, energy: float, rest_energy: float, slots: np.ndarray,
+                      coeffs: np.ndarray, grad_ptr: int, nparams: int,
+                      stream: int | None = None) -> None:
+        """Independent blocks of turns in one launch (atx_adjoint_batch).
+        items: sequence of (states_ptr, lam_ptr, nturns, turn, npart,
+        accumulate)."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        coeffs = np.ascontiguousarray(coeffs, np.int32)
+        if slots.size != self.nelem or coeffs.size != self.nelem:
+            raise ValueError("grad_slot/grad_coeff need one entry per element")
+        arr = (AdjointItem * max(len(items), 1))()
+        for i, (sp, lp, nt, t, n, acc) in enumerate(items):
+            arr[i] = AdjointItem(sp, lp, int(nt), int(t), int(n), 1 if acc else 0)
+        ip = C.POINTER(C.c_int32)
+        _check(self.lib.atx_adjoint_batch(self.h, arr, len(items), float(energy),
+                                          float(rest_energy), slots.ctypes.data_as(ip),
+                                          coeffs.ctypes.data_as(ip), grad_ptr, int(nparams),
+                                          stream), self.lib)
 
     def diffusion_matrices(self, orbits: np.ndarray, energy: float,
                            rest_energy: float) -> np.ndarray:

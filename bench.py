@@ -330,7 +330,7 @@ def bench_adjoint(args) -> dict:
     import torch
 
     from at_amd import lattice as L
-    from at_amd.adjoint import Problem, TrackingLoss
+    from at_amd.adjoint import Problem, TrackingLoss, use_parallel_sweep
     from at_amd.workloads import gaussian_beam, large_ring
 
     ring = large_ring(radiation=False, cavity=True)
@@ -370,6 +370,22 @@ def bench_adjoint(args) -> dict:
     wall = time.perf_counter() - t0
     ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     fms = float(np.mean([a.elapsed_time(b) for a, b in fw]))
+    sweep = "parallel" if use_parallel_sweep(n, (T + 7) // 8, 4 * torch.cuda.get_device_properties(
+        dev).multi_processor_count) else "sequential"
+    seq_ms = None
+    if not args.no_compare and sweep == "parallel":
+        # the same gradient with the blocks swept one after the other
+        prob.parallel = False
+        one()
+        torch.cuda.synchronize()
+        sq = _events(K)
+        for k in range(K):
+            sq[k][0].record(stream)
+            one()
+            sq[k][1].record(stream)
+        torch.cuda.synchronize()
+        seq_ms = float(np.mean([a.elapsed_time(b) for a, b in sq]))
+        prob.parallel = None
     out = {
         "metric": "strength gradients: gradient evaluations/s (forward + adjoint)",
         "value": 1e3 / ms,
@@ -381,7 +397,8 @@ def bench_adjoint(args) -> dict:
                                f"{n} particles, {T} turns, {len(params)} strengths "
                                f"(PolynomB[1] of quadrupoles, PolynomB[2] of sextupoles), "
                                f"checkpoints every 8 turns",
-                   "forward_ms": fms, "gradient_ms": ms,
+                   "forward_ms": fms, "gradient_ms": ms, "sweep": sweep,
+                   "sequential_sweep_gradient_ms": seq_ms,
                    "adjoint_over_forward": (ms - fms) / fms},
     }
     if not args.no_cpu:

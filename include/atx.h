@@ -165,6 +165,27 @@ int atx_adjoint(atx_lattice *lat, const double *states, int32_t nturns, int32_t 
                 const int32_t *grad_slot, const int32_t *grad_coeff, double *grad,
                 int32_t num_params, void *stream);
 
+/* Several independent blocks of the same lattice in one launch (no
+ * counterpart in the reference). Each item is one atx_adjoint call: its
+ * own turn-start states, adjoint (in place), first turn and particle
+ * count; `accumulate` nonzero adds its dL/dPolynomB to grad, zero only
+ * propagates lam (e.g. unit seeds, giving rows of the block's transposed
+ * Jacobian). grad may be NULL when no element has a gradient slot. At most
+ * 65,535 items. Asynchronous on stream. */
+typedef struct atx_adjoint_item {
+  const double *states; /* device, (nturns+1) (6,N) blocks */
+  double *lam;          /* device (6,N), in place */
+  int32_t nturns;
+  int32_t turn;
+  uint32_t num_particles;
+  int32_t accumulate;
+} atx_adjoint_item;
+
+int atx_adjoint_batch(atx_lattice *lat, const atx_adjoint_item *items, int32_t nitems,
+                      double energy, double rest_energy, const int32_t *grad_slot,
+                      const int32_t *grad_coeff, double *grad, int32_t num_params,
+                      void *stream);
+
 /* --- Diffusion matrices --------------------------------------------------
  * Replaces at_diffmatrix (pyat/at.c:755-831), called once per element by
  * ohmi_envelope (pyat/at/physics/radiation.py:78-121): for every element of

@@ -137,6 +137,14 @@ int main() {
     EXPECT(atx_lattice_check(e, 3e9, 0.0) != ATX_OK);
     EXPECT(atx_element_set_array(e, "T1", nullptr, 6) != ATX_OK);
     EXPECT(atx_element_set_double(nullptr, "Length", 1.0) != ATX_OK);
+    // adjoint item lists are validated before any device work
+    atx_adjoint_item it{};
+    it.nturns = 1;
+    it.num_particles = 10;
+    EXPECT(atx_adjoint_batch(e, &it, 1, 3e9, 0.0, nullptr, nullptr, nullptr, 0, nullptr) != ATX_OK);
+    EXPECT(atx_adjoint_batch(e, nullptr, 2, 3e9, 0.0, nullptr, nullptr, nullptr, 0, nullptr) != ATX_OK);
+    EXPECT(atx_adjoint_batch(e, &it, 70000, 3e9, 0.0, nullptr, nullptr, nullptr, 0, nullptr) != ATX_OK);
+    EXPECT(atx_adjoint_batch(nullptr, &it, 1, 3e9, 0.0, nullptr, nullptr, nullptr, 0, nullptr) != ATX_OK);
     EXPECT(atx_lattice_destroy(e) == ATX_OK);
     EXPECT(atx_lattice_destroy(nullptr) == ATX_OK);
   }

@@ -135,5 +135,45 @@ def test_batch_argument_errors():
     assert rc == engine.ATX_E_VALUE
 
 
+def test_adjoint_batch_argument_errors():
+    """atx_adjoint_batch rejects bad item lists before any device work."""
+    lib = engine.load_library()
+    lat = engine.DeviceLattice([Drift("d", 1.0)])
+    slots = np.full(1, -1, np.int32)
+    ip = C.POINTER(C.c_int32)
+    sp, cp = slots.ctypes.data_as(ip), slots.ctypes.data_as(ip)
+
+    def call(items, n=None, grad=None):
+        arr = (engine.AdjointItem * max(len(items), 1))(*items)
+        return lib.atx_adjoint_batch(lat.h, arr, len(items) if n is None else n, 3e9, 0.0, sp,
+                                     cp, grad, 0, None)
+
+    assert lib.atx_adjoint_batch(None, None, 0, 3e9, 0.0, None, None, None, 0, None) \
+        == engine.ATX_E_VALUE
+    assert call([], n=-1) == engine.ATX_E_VALUE
+    assert call([], n=70000) == engine.ATX_E_VALUE
+    # particles but NULL arrays
+    assert call([engine.AdjointItem(None, None, 1, 0, 10, 0)]) == engine.ATX_E_VALUE
+    # negative turn count
+    assert call([engine.AdjointItem(8, 8, -1, 0, 10, 0)]) == engine.ATX_E_VALUE
+    # accumulate without a gradient array
+    assert call([engine.AdjointItem(8, 8, 1, 0, 10, 1)]) == engine.ATX_E_VALUE
+    assert "grad is NULL" in lib.atx_last_error().decode()
+
+
+def test_parallel_adjoint_sweep_cost_model():
+    """The block-parallel sweep is chosen only while the beam leaves the
+    device's SIMDs idle."""
+    from at_amd.adjoint import use_parallel_sweep
+
+    simds = 1024
+    assert use_parallel_sweep(4096, 8, simds)        # 64 waves on 1,024 SIMDs
+    assert use_parallel_sweep(512, 22, simds)
+    assert not use_parallel_sweep(4096, 1, simds)    # one block: nothing to split
+    assert not use_parallel_sweep(4096, 2, simds)    # two launches either way
+    assert not use_parallel_sweep(65536, 8, simds)   # one wave per SIMD already
+    assert not use_parallel_sweep(262144, 125, simds)
+
+
 def test_device_count_without_gpu_is_zero_or_more():
     assert engine.device_count() >= 0

@@ -11,6 +11,11 @@
   finite differences (h = 1e-7*|K|) of the compiled reference tracking.
 * Size-independent checks at the full 4,096-particle size: the gradient is
   additive over particle subsets, and independent of the checkpoint spacing.
+* atx_adjoint_batch: one item equals atx_adjoint bit for bit (adjoint) and
+  to rounding (gradient, accumulated with atomics); items of different
+  sizes in one launch equal their own calls; the block-parallel sweep
+  (unit seeds through the blocks, then one launch over all blocks) equals
+  the sequential one.
 """
 
 import copy
@@ -143,7 +148,7 @@ def test_tangent_maps_equal_fd_of_oracle(name):
         assert err <= 2e-8 * scale, rows[ip] - jac
 
 
-def _ring_problem(checkpoint=8):
+def _ring_problem(checkpoint=8, parallel=None):
     ring = large_ring(radiation=False, cavity=True)
     params = []
     for i, e in enumerate(ring):
@@ -151,7 +156,7 @@ def _ring_problem(checkpoint=8):
             params.append((i, 1))
         elif e.PassMethod.startswith("StrMPole") and e.MaxOrder == 2:
             params.append((i, 2))
-    return ring, Problem(ring, params, checkpoint=checkpoint)
+    return ring, Problem(ring, params, checkpoint=checkpoint, parallel=parallel)
 
 
 def _loss_cpu(ring, beam, nturns, edits, impl):
@@ -228,3 +233,88 @@ def test_adjoint_rejects_radiating_lattice():
     prob = Problem(ring, [(idx, 1)])
     with pytest.raises(RuntimeError, match="adjoint not available"):
         loss_and_grad(prob, gaussian_beam(64, seed=1), 2)
+
+
+def _block_states(prob, beam, nturns, turn0):
+    """Turn-start states of one block and its end state, (nturns+1, N, 6)."""
+    import torch
+
+    r = torch.from_numpy(np.ascontiguousarray(beam.T)).to(torch.device("cuda", 0))
+    n = r.shape[0]
+    st = torch.empty((nturns + 1, n, 6), dtype=torch.float64, device=r.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    for t in range(nturns):
+        st[t].copy_(r)
+        prob.dl.track_device(r.data_ptr(), n, 1, turn=turn0 + t, energy=prob.energy,
+                             rest_energy=prob.rest, stream=stream)
+    st[nturns].copy_(r)
+    return st
+
+
+def test_adjoint_batch_items_equal_single_calls():
+    import torch
+
+    ring, prob = _ring_problem()
+    npar = len(prob.params)
+    n, nturns, turn0 = 1000, 3, 5
+    beam = gaussian_beam(n, seed=21)
+    st = _block_states(prob, beam, nturns, turn0)
+    assert torch.isnan(st[-1][:, 0]).sum().item() < n  # some particles survive
+    gen = torch.Generator().manual_seed(3)
+    lam0 = torch.randn((n, 6), generator=gen, dtype=torch.float64).to(st.device)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def single(lam, grad, states=st, npart=n):
+        prob.dl.adjoint(states.data_ptr(), nturns, turn0, npart, prob.energy, prob.rest,
+                        lam.data_ptr(), prob.slots, prob.coeffs, grad.data_ptr(), npar,
+                        stream=stream)
+
+    lam_a = lam0.clone()
+    grad_a = torch.zeros(npar, dtype=torch.float64, device=st.device)
+    single(lam_a, grad_a)
+    # one item
+    lam_b = lam0.clone()
+    grad_b = torch.zeros_like(grad_a)
+    prob.dl.adjoint_batch([(st.data_ptr(), lam_b.data_ptr(), nturns, turn0, n, True)],
+                          prob.energy, prob.rest, prob.slots, prob.coeffs, grad_b.data_ptr(),
+                          npar, stream=stream)
+    assert torch.equal(lam_a, lam_b)
+    gmax = grad_a.abs().max().item()
+    assert gmax > 0
+    assert (grad_a - grad_b).abs().max().item() <= 1e-13 * gmax
+    # three items in one launch: the block again (accumulating), the block
+    # with another adjoint (not accumulating), and the first 300 particles
+    # of the block (states of their own, not accumulating)
+    lam_c = lam0.clone()
+    lam_d = (2.0 * lam0 + 1.0).contiguous()
+    lam_e = lam0[:300].clone()
+    st_e = st[:, :300].contiguous()
+    grad_c = torch.zeros_like(grad_a)
+    prob.dl.adjoint_batch([(st.data_ptr(), lam_c.data_ptr(), nturns, turn0, n, True),
+                           (st.data_ptr(), lam_d.data_ptr(), nturns, turn0, n, False),
+                           (st_e.data_ptr(), lam_e.data_ptr(), nturns, turn0, 300, False)],
+                          prob.energy, prob.rest, prob.slots, prob.coeffs, grad_c.data_ptr(),
+                          npar, stream=stream)
+    lam_d_ref = (2.0 * lam0 + 1.0).contiguous()
+    scratch = torch.zeros_like(grad_a)
+    single(lam_d_ref, scratch)
+    assert torch.equal(lam_c, lam_a)
+    assert torch.equal(lam_d, lam_d_ref)
+    assert torch.equal(lam_e, lam_a[:300])
+    assert (grad_c - grad_a).abs().max().item() <= 1e-13 * gmax
+
+
+@pytest.mark.parametrize("checkpoint", [8, 5])
+def test_parallel_sweep_equals_sequential(checkpoint):
+    ring, pseq = _ring_problem(checkpoint=checkpoint, parallel=False)
+    _, ppar = _ring_problem(checkpoint=checkpoint, parallel=True)
+    beam = gaussian_beam(4096, seed=8)
+    nturns = 64
+    l_seq, g_seq = loss_and_grad(pseq, beam, nturns)
+    l_par, g_par = loss_and_grad(ppar, beam, nturns)
+    gmax = np.abs(g_seq).max()
+    err = np.abs(g_par - g_seq).max()
+    print(f"checkpoint {checkpoint}: loss {l_seq!r}, max |g_par - g_seq| = {err:.3e} "
+          f"of max |g| {gmax:.3e}", flush=True)
+    assert l_par == l_seq
+    assert err <= 1e-11 * gmax
