@@ -252,6 +252,12 @@ class Server:
             raise RuntimeError("server is not running")
         if self._error is not None:
             raise RuntimeError("server stopped on an error") from self._error
+        if self.need(req) > self.budget:
+            # rejected here: admitted, it could never fit and would stop the server
+            raise ValueError(f"request {req.rid} needs {self.need(req)} bytes, over the HBM "
+                             f"budget of {self.budget}")
+        if req.rid in self._futures:
+            raise ValueError(f"request id {req.rid} is already being served")
         fut = Future()
         self._futures[req.rid] = fut
         self._inbox.put((req, fut, time.perf_counter() - self._t0))
@@ -264,11 +270,15 @@ class Server:
         self._cancel.add(int(rid))
 
     def stop(self, timeout: float | None = None) -> None:
-        """Serve what was submitted, then end the background thread."""
+        """Serve what was submitted, then end the background thread
+        (TimeoutError if it is still serving after ``timeout`` seconds;
+        ``stop`` may then be called again)."""
         if getattr(self, "_thread", None) is None:
             return
         self._stopping.set()
         self._thread.join(timeout)
+        if self._thread.is_alive():
+            raise TimeoutError("server still serving submitted requests")
         self._thread = None
         if self._error is not None:
             raise RuntimeError("server stopped on an error") from self._error
@@ -495,7 +505,8 @@ def alive_passes(requests, results, nelem_of) -> int:
     for q, r in zip(requests, results):
         ne = nelem_of(q.instance)
         lost = r.lost["islost"]
-        total += int((~lost).sum()) * q.nturns * ne
+        nt = r.turns_done if r.turns_done >= 0 else q.nturns  # ended early: turns tracked
+        total += int((~lost).sum()) * nt * ne
         if lost.any():
             t = r.lost["turn"][lost].astype(np.int64)
             e = r.lost["elem"][lost].astype(np.int64)

@@ -164,6 +164,12 @@ def test_online_submit_from_threads_equals_alone():
     for t in threads:
         t.join()
     results = {rid: f.result(timeout=120) for rid, f in futs.items()}
+    # a request over the HBM budget is refused at submit; the server goes on
+    with pytest.raises(ValueError, match="over the HBM budget"):
+        srv.submit(Request(999, 0, gaussian_beam(5001, seed=1), 4))
+    q0 = trace[0]
+    again = srv.submit(Request(1000, q0.instance, q0.particles, q0.nturns)).result(timeout=120)
+    np.testing.assert_array_equal(again.particles, results[q0.rid].particles)
     srv.stop()
     with pytest.raises(RuntimeError):
         srv.submit(trace[0])
