@@ -17,10 +17,13 @@
 #include "../../include/atx.h"
 #include "atx_device.h"
 
-extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, int, hipStream_t);
-extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, int, const int32_t*,
+                                              hipStream_t);
+extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, const int32_t*,
+                                            hipStream_t);
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
 extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
+extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int32_t*, hipStream_t);
 extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, int32_t, hipStream_t);
 
 namespace atx {
@@ -694,9 +697,9 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   hipError_t le = hipSuccess;
   if (ce == hipSuccess)
     le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
-                                          ext ? 1 : 0, stream)
+                                          ext ? 1 : 0, nullptr, stream)
                 : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
-                                        ext ? 1 : 0, stream);
+                                        ext ? 1 : 0, nullptr, stream);
   hipError_t fe = hipFreeAsync(d_scratch, stream);
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
@@ -738,7 +741,8 @@ std::vector<int> repack_bounds(int nturns) {
 }
 
 // Batched launch in turn blocks with the survivors of every instance packed
-// into its leading workgroups between blocks (repack_kernel). All block
+// into its leading slots between blocks (repack_kernel) and the workgroups
+// holding them mapped to the front of the grid (live_map_kernel). All block
 // tables go up in one copy; the track and repack launches are captured on a
 // private stream into one graph, which runs on `stream`. Each particle's
 // arithmetic is unchanged, so results equal those of launch_instances.
@@ -782,6 +786,8 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   bytes += (size_t)nidx * sizeof(int32_t);
   const size_t nlive_off = bytes;
   bytes += ninst * sizeof(int32_t);
+  const size_t live_off = bytes;  // live workgroup map of the current block
+  bytes += (ninst + 1) * sizeof(int32_t);
   char* d = nullptr;
   HIPCHK(hipMallocAsync((void**)&d, bytes, stream));
   std::vector<char> host(upload);
@@ -814,10 +820,15 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   if (ge == hipSuccess) ge = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
   if (ge == hipSuccess) {
     hipError_t le = hipSuccess;
+    int32_t* live = (int32_t*)(d + live_off);
     for (int k = 0; k < nblk && le == hipSuccess; k++) {
       const Instance* t = (const Instance*)(d + (size_t)k * tab);
-      le = strict ? atx_launch_track_strict(t, (int)ninst, nblocks, block, ext ? 1 : 0, cap)
-                  : atx_launch_track_fast(t, (int)ninst, nblocks, block, ext ? 1 : 0, cap);
+      // after the first block the survivors' workgroups lead the grid
+      if (k > 0) le = atx_launch_live_map(t, (int)ninst, block, live, cap);
+      if (le != hipSuccess) break;
+      const int32_t* lb = k > 0 ? live : nullptr;
+      le = strict ? atx_launch_track_strict(t, (int)ninst, nblocks, block, ext ? 1 : 0, lb, cap)
+                  : atx_launch_track_fast(t, (int)ninst, nblocks, block, ext ? 1 : 0, lb, cap);
       if (le == hipSuccess && k + 1 < nblk)
         le = atx_launch_repack((const Instance*)(d + (size_t)(k + 1) * tab), (int)ninst, k == 0 ? 1 : 0, cap);
     }

@@ -1229,16 +1229,33 @@ __device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
   rp[5 * cs] = r.ct;
 }
 
+// live_begin (repacked turn blocks after the first; NULL otherwise): first
+// workgroup of each instance among the workgroups that hold survivors, with
+// live_begin[ninst] their total, so that the live workgroups of all
+// instances are the leading ones of the grid and the hardware spreads them
+// evenly over the CUs; the other workgroups end at once.
 template <bool EXT>
-__global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst) {
+__global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst,
+                                                    const int32_t* __restrict__ live_begin) {
   extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
   // Instance of this workgroup: the last whose first block is <= blockIdx.x.
   int lo = 0, hi = ninst - 1;
   const int bid = (int)blockIdx.x;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (insts[mid].block_begin <= bid) lo = mid; else hi = mid - 1;
+  int wbegin;
+  if (live_begin) {
+    if (bid >= live_begin[ninst]) return;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (live_begin[mid] <= bid) lo = mid; else hi = mid - 1;
+    }
+    wbegin = live_begin[lo];
+  } else {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (insts[mid].block_begin <= bid) lo = mid; else hi = mid - 1;
+    }
+    wbegin = insts[lo].block_begin;
   }
   cInstance& I = insts[lo];
   const int tid = (int)threadIdx.x;
@@ -1247,7 +1264,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   // lanes of a repacked launch take the surviving particles idx[0..nlive)
   const int32_t* idx = I.idx;
   const int nslot = idx ? __builtin_amdgcn_readfirstlane(*I.nlive) : npart;
-  const int first = (bid - I.block_begin) * B;  // first slot of this workgroup
+  const int first = (bid - wbegin) * B;  // first slot of this workgroup
   // waves of this workgroup that hold particles (uniform); the others end now
   int nw = (min(B, nslot - first) + 63) >> 6;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1391,13 +1408,13 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 }  // namespace atx
 
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
-                                 int ext, hipStream_t stream) {
+                                 int ext, const int32_t* live_begin, hipStream_t stream) {
   if (ext)
     hipLaunchKernelGGL(atx::ATX_NS::track_kernel<true>, dim3(nblocks), dim3(block),
-                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
+                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst, live_begin);
   else
     hipLaunchKernelGGL(atx::ATX_NS::track_kernel<false>, dim3(nblocks), dim3(block),
-                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst);
+                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst, live_begin);
   return hipGetLastError();
 }
 
@@ -1447,10 +1464,62 @@ __global__ void __launch_bounds__(1024) repack_kernel(const Instance* __restrict
 }
 }  // namespace atx
 
+namespace atx {
+// Map of the live workgroups for the next turn block (one workgroup):
+// live_begin[i] = sum over j < i of ceil(nlive_j / block) for the instances
+// with turns in the block (none for the others), live_begin[ninst] = total.
+// Chunks of 1,024 instances: inclusive scan in each wave, then the wave
+// totals through LDS, carrying the running sum from chunk to chunk.
+__global__ void __launch_bounds__(1024) live_map_kernel(const Instance* __restrict__ insts, int ninst,
+                                                        int block, int32_t* __restrict__ live_begin) {
+  __shared__ int s_wave[16];
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = (int)blockDim.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < ninst; base += (int)blockDim.x) {
+    const int i = base + tid;
+    int c = 0;
+    if (i < ninst) {
+      const Instance& I = insts[i];
+      const int n = I.idx ? *I.nlive : I.npart;
+      c = I.nturns > 0 ? (n + block - 1) / block : 0;
+    }
+    int v = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(v, off, 64);
+      if (lane >= off) v += u;
+    }
+    if (lane == 63) s_wave[wave] = v;
+    __syncthreads();
+    int woff = 0, total = 0;
+    for (int k = 0; k < nwaves; k++) {
+      const int w = s_wave[k];
+      woff += (k < wave) ? w : 0;
+      total += w;
+    }
+    if (i < ninst) live_begin[i] = carry + woff + v - c;
+    carry += total;
+    __syncthreads();
+  }
+  if (tid == 0) live_begin[ninst] = carry;
+}
+}  // namespace atx
+
 extern "C" hipError_t atx_launch_repack(const atx::Instance* d_insts, int ninst, int first_pass,
                                         hipStream_t stream) {
   if (ninst <= 0) return hipSuccess;
   hipLaunchKernelGGL(atx::repack_kernel, dim3(ninst), dim3(1024), 0, stream, d_insts, first_pass);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t atx_launch_live_map(const atx::Instance* d_insts, int ninst, int block,
+                                          int32_t* live_begin, hipStream_t stream) {
+  if (ninst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::live_map_kernel, dim3(1), dim3(1024), 0, stream, d_insts, ninst, block,
+                     live_begin);
   return hipGetLastError();
 }
 

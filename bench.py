@@ -82,8 +82,10 @@ def _events(n):
 def bench_batch(args) -> dict:
     """Batched error realisations: ``--instances`` error-seeded copies of the
     large ring, each with a DA polar grid of U[1,000, 40,000] particles, all
-    tracked for ``--batch-turns`` turns in ONE launch (workgroup size from
-    the autotuner), against the same instances tracked one launch each."""
+    tracked for ``--batch-turns`` turns (1,024 by default) in ONE launch
+    (workgroup size from the autotuner), against the same launch without
+    repacking and, with ``--per-instance``, the instances tracked one launch
+    each."""
     import numpy as np
     import torch
 
@@ -104,6 +106,8 @@ def bench_batch(args) -> dict:
     tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False)
     block = tune["block_size"]
     T = args.batch_turns
+    print(f"batch: autotuned workgroup {block}; {len(dls)} instances, {T} turns",
+          file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
     parts = [b.clone() for b in base]
     (e0, e1), = _events(1)
@@ -116,6 +120,7 @@ def bench_batch(args) -> dict:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
+    print(f"batch: launch {ms:.1f} ms", file=sys.stderr, flush=True)
     # the same launch without repacking between turn blocks
     plain_ms = plain_same = None
     if not args.no_repack and not args.no_compare:
@@ -127,6 +132,7 @@ def bench_batch(args) -> dict:
         p1.record(stream)
         torch.cuda.synchronize()
         plain_ms = p0.elapsed_time(p1)
+        print(f"batch: plain launch {plain_ms:.1f} ms", file=sys.stderr, flush=True)
         plain_same = all(torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0))
                          for a, b in zip(parts, parts3))
     flops = passes = 0.0
@@ -137,7 +143,7 @@ def bench_batch(args) -> dict:
         flops += fl
         passes += pa
     seq_ms = None
-    if not args.no_compare:
+    if not args.no_compare and args.per_instance:
         parts2 = [b.clone() for b in base]
         (s0, s1), = _events(1)
         torch.cuda.synchronize()
@@ -430,7 +436,7 @@ def main() -> int:
                          "trace; adjoint: strength gradients; monitors: refpts at every "
                          "element (HBM bandwidth of the copies)")
     ap.add_argument("--instances", type=int, default=64)
-    ap.add_argument("--batch-turns", type=int, default=256)
+    ap.add_argument("--batch-turns", type=int, default=1024)
     ap.add_argument("--no-repack", action="store_true",
                     help="batch/serve: one plain launch instead of repacked turn blocks")
     ap.add_argument("--requests", type=int, default=12)
@@ -439,6 +445,9 @@ def main() -> int:
     ap.add_argument("--adj-particles", type=int, default=4096)
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--per-instance", action="store_true",
+                    help="batch: also time one launch per instance (about 6 minutes at 1,024 "
+                         "turns)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--turns-per-step", type=int, default=100)

@@ -85,17 +85,22 @@ def test_repacked_launch_equals_plain(losses, block):
     """Turn blocks with the survivors repacked between them (one graph):
     coordinates and loss maps equal the plain launch bit for bit, on every
     call (the graph is rebuilt and replayed per call), with per-item turn
-    counts that end in different blocks."""
+    counts that end in different blocks and an item (third of five) that
+    loses every particle on its first turn, so that later blocks map no
+    workgroup to it or to the finished items."""
     import torch
 
     base = large_ring(ncells=2, cavity=True)
-    lats = [error_realisation(base, 300 + i) for i in range(4)]
+    lats = [error_realisation(base, 300 + i) for i in range(5)]
     dls = [engine.DeviceLattice(l) for l in lats]
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
     dev = torch.device("cuda:0")
-    sizes = [2000, 700, 1500, 333]
+    sizes = [2000, 700, 600, 1500, 333]
     parts = [da_polar_grid(n, xmax=12e-3, ymax=5e-3) for n in sizes]
-    turns = [(37, 0), (64, 5), (9, 1), (1, 0)]
+    gone = np.zeros((6, 600))
+    gone[0] = np.linspace(0.05, 0.08, 600)  # outside the 10 mm apertures
+    parts[2] = np.asfortranarray(gone)
+    turns = [(37, 0), (64, 5), (40, 0), (9, 1), (1, 0)]
     plain = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
     lm_plain = track_batch(dls, plain, 0, energy=energy, rest_energy=rest, losses=losses,
                            block_size=block, item_turns=turns)
