@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -657,6 +658,49 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
 
 constexpr int kBlock = 256;
 
+// Pinned staging for the launch tables (instance tables, refpt slot tables,
+// adjoint tables). hipMemcpyAsync from pageable memory returns only once the
+// work queued before it on the stream has run, which would hold the host
+// (and the streams it feeds next) behind every running kernel; from pinned
+// memory it returns at once. A staging buffer is reused when the event
+// recorded after its last copy has completed; buffers live until the
+// process ends (at most one per upload in flight at a time).
+struct Staging {
+  char* p = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+};
+std::mutex g_stage_mu;
+std::vector<std::unique_ptr<Staging>> g_stage;
+
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return hipSuccess;
+  std::lock_guard<std::mutex> g(g_stage_mu);
+  Staging* st = nullptr;
+  for (auto& x : g_stage)
+    if (x->cap >= bytes && hipEventQuery(x->ev) == hipSuccess) {
+      st = x.get();
+      break;
+    }
+  if (!st) {
+    auto n = std::make_unique<Staging>();
+    n->cap = std::max(bytes, (size_t)(64 << 10));
+    hipError_t e = hipHostMalloc((void**)&n->p, n->cap, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    e = hipEventCreateWithFlags(&n->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipHostFree(n->p);
+      return e;
+    }
+    st = n.get();
+    g_stage.push_back(std::move(n));
+  }
+  std::memcpy(st->p, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, st->p, bytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return e;
+  return hipEventRecord(st->ev, stream);
+}
+
 // Launch one kernel over several instances: instances get consecutive
 // workgroup ranges; one stream-ordered scratch block carries the instance
 // table and every instance's refpt slot table.
@@ -693,7 +737,7 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   for (size_t i = 0; i < insts.size(); i++) insts[i].slots = (const int32_t*)(d_scratch + soff[i]);
   for (auto& c : copies) std::memcpy(host.data() + c.first, c.second->data(), c.second->size() * sizeof(int32_t));
   std::memcpy(host.data(), insts.data(), insts.size() * sizeof(Instance));
-  hipError_t ce = hipMemcpyAsync(d_scratch, host.data(), bytes, hipMemcpyHostToDevice, stream);
+  hipError_t ce = upload_async(d_scratch, host.data(), bytes, stream);
   hipError_t le = hipSuccess;
   if (ce == hipSuccess)
     le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
@@ -701,11 +745,9 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
                 : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
                                         ext ? 1 : 0, nullptr, stream);
   hipError_t fe = hipFreeAsync(d_scratch, stream);
-  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("upload: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
   if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
-  // hipMemcpyAsync from pageable memory returns once the source is staged,
-  // so `host` may go out of scope here.
   return ATX_OK;
 }
 
@@ -810,7 +852,7 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
       std::memcpy(host.data() + (size_t)k * tab + i * sizeof(Instance), &in, sizeof(Instance));
     }
   }
-  hipError_t ce = hipMemcpyAsync(d, host.data(), upload, hipMemcpyHostToDevice, stream);
+  hipError_t ce = upload_async(d, host.data(), upload, stream);
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1245,7 +1287,7 @@ static int adjoint_launch(atx_lattice* lat, const std::vector<AdjItem>& items, d
   hipStream_t stream = (hipStream_t)stream_;
   unsigned char* d_buf = nullptr;
   HIPCHK(hipMallocAsync((void**)&d_buf, host.size(), stream));
-  hipError_t ce = hipMemcpyAsync(d_buf, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+  hipError_t ce = upload_async(d_buf, host.data(), host.size(), stream);
   hipError_t le = hipSuccess;
   if (ce == hipSuccess) {
     AdjArgs a;
@@ -1292,6 +1334,7 @@ int atx_adjoint_batch(atx_lattice* lat, const atx_adjoint_item* items, int32_t n
   if (nitems > 65535) return fail(ATX_E_VALUE, "more than 65535 items");
   std::vector<AdjItem> v;
   v.reserve((size_t)nitems);
+  bool accumulate = false;
   for (int32_t i = 0; i < nitems; i++) {
     const atx_adjoint_item& s = items[i];
     if (s.nturns < 0) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": nturns must be >= 0");
@@ -1304,13 +1347,15 @@ int atx_adjoint_batch(atx_lattice* lat, const atx_adjoint_item* items, int32_t n
     it.states = s.states;
     it.lam = s.lam;
     it.grad = s.accumulate ? grad : nullptr;
+    accumulate = accumulate || s.accumulate;
     it.npart = (int32_t)s.num_particles;
     it.nturns = s.nturns;
     it.turn0 = s.turn;
     v.push_back(it);
   }
-  return adjoint_launch(lat, v, energy, rest_energy, grad_slot, grad_coeff, grad != nullptr, num_params,
-                        stream_);
+  // gradient slots need a gradient array only when some item accumulates
+  return adjoint_launch(lat, v, energy, rest_energy, grad_slot, grad_coeff,
+                        grad != nullptr || !accumulate, num_params, stream_);
 }
 
 int atx_diffusion_matrices(atx_lattice* lat, const double* orbits, double* bdiff, double energy,

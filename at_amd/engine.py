@@ -240,7 +240,7 @@ class DeviceLattice:
                       stream: int | None = None) -> None:
         """Independent blocks of turns in one launch (atx_adjoint_batch).
         items: sequence of (states_ptr, lam_ptr, nturns, turn, npart,
-        accumulate)."""
+        accumulate); grad_ptr may be None when no item accumulates."""
         slots = np.ascontiguousarray(slots, np.int32)
         coeffs = np.ascontiguousarray(coeffs, np.int32)
         if slots.size != self.nelem or coeffs.size != self.nelem:

@@ -46,8 +46,10 @@ extern "C" {
                                      particles already flagged are not recorded again */
 #define ATX_REPACK 0x20u         /* atx_track_batch: track in turn blocks of 1, 1, 2, 4, ... turns
                                   * and, between blocks, pack each instance's surviving
-                                  * particles into its leading workgroups; the blocks run as
-                                  * one hipGraph. Results are identical to a plain launch. */
+                                  * particles into its leading slots and the workgroups that
+                                  * hold them, over all instances, at the front of the grid;
+                                  * the blocks run as one hipGraph. Results are identical to
+                                  * a plain launch. */
 
 typedef struct atx_lattice atx_lattice;
 
@@ -170,8 +172,8 @@ int atx_adjoint(atx_lattice *lat, const double *states, int32_t nturns, int32_t 
  * own turn-start states, adjoint (in place), first turn and particle
  * count; `accumulate` nonzero adds its dL/dPolynomB to grad, zero only
  * propagates lam (e.g. unit seeds, giving rows of the block's transposed
- * Jacobian). grad may be NULL when no element has a gradient slot. At most
- * 65,535 items. Asynchronous on stream. */
+ * Jacobian). grad may be NULL when no item accumulates or no element has a
+ * gradient slot. At most 65,535 items. Asynchronous on stream. */
 typedef struct atx_adjoint_item {
   const double *states; /* device, (nturns+1) (6,N) blocks */
   double *lam;          /* device (6,N), in place */

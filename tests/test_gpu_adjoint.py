@@ -302,6 +302,12 @@ def test_adjoint_batch_items_equal_single_calls():
     assert torch.equal(lam_d, lam_d_ref)
     assert torch.equal(lam_e, lam_a[:300])
     assert (grad_c - grad_a).abs().max().item() <= 1e-13 * gmax
+    # no item accumulates: no gradient array needed, gradient slots or not
+    lam_f = (2.0 * lam0 + 1.0).contiguous()
+    prob.dl.adjoint_batch([(st.data_ptr(), lam_f.data_ptr(), nturns, turn0, n, False)],
+                          prob.energy, prob.rest, prob.slots, prob.coeffs, None, npar,
+                          stream=stream)
+    assert torch.equal(lam_f, lam_d_ref)
 
 
 @pytest.mark.parametrize("checkpoint", [8, 5])
