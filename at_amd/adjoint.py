@@ -21,13 +21,15 @@ backward is the hand-written adjoint kernel (atx_adjoint):
   matrix per particle) together with the last block's own sweep; lambda at
   every block boundary then follows by 6x6 products, and a second launch
   sweeps the remaining blocks together, accumulating the gradient. That is
-  two launches of 8 turns instead of nblocks in sequence, for 7 times the
-  lane work.
+  two launches instead of nblocks in sequence, for about 7 times the lane
+  work. The parallel items are sub-blocks of g turns of the checkpoint
+  blocks (their turn-start states are recorded), g chosen by a cost model
+  (sweep_plan).
 
 Memory: checkpoints (nturns/checkpoint + 1) x N x 48 B plus the turn-start
 states of all blocks, (nturns + nblocks) x N x 48 B (12.6 MB for 4,096
-particles and 64 turns); the parallel sweep adds (nblocks - 1) x N x 288 B
-of block matrices.
+particles and 64 turns); the parallel sweep adds (S - 1) x N x 288 B of
+sub-block matrices for S sub-blocks.
 """
 
 from __future__ import annotations
@@ -42,10 +44,12 @@ class Problem:
 
     params: sequence of (element index, PolynomB index). parallel: sweep
     the blocks in parallel (True), in sequence (False) or as a cost model of
-    the device's occupancy prefers (None)."""
+    the device's occupancy prefers (None). sub_block: turns per item of the
+    parallel sweep (None: chosen with the sweep by the cost model; 0:
+    sequential)."""
 
     def __init__(self, lattice, params, *, energy=None, rest_energy=None, checkpoint: int = 8,
-                 parallel: bool | None = None):
+                 parallel: bool | None = None, sub_block: int | None = None):
         self.lattice = lattice
         self.params = [(int(e), int(k)) for e, k in params]
         self.energy = float(lattice.energy if energy is None else energy)
@@ -53,6 +57,7 @@ class Problem:
         self.rest = float(getattr(part, "rest_energy", 0.0) if rest_energy is None else rest_energy)
         self.checkpoint = int(checkpoint)
         self.parallel = parallel
+        self.sub_block = None if sub_block is None else int(sub_block)
         self.dl = engine.DeviceLattice(lattice)
         n = len(lattice)
         self.slots = np.full(n, -1, np.int32)
@@ -78,23 +83,43 @@ class Problem:
 
 
 def _sweep_cost(lanes: int, simds: int) -> float:
-    """Time of one sweep over `lanes` particles, in units of a sweep with at
-    most one wave per SIMD: below that the dependent FP64 chain of each
-    lane sets the time; above it each further wave per SIMD adds about 0.64
-    of a wave's time (the tracking kernel takes 2.57 times as long with 4
-    waves per SIMD as with 1)."""
-    return max(1.0, 0.64 * lanes / (64.0 * simds))
+    """Time per turn of one adjoint launch over `lanes` particles, in units
+    of a launch with at most one wave per SIMD: below that the dependent FP64
+    chain of each lane sets the time; above it each further wave per SIMD
+    adds about 0.73 of a wave's time (measured on the large ring: 43 blocks
+    of 4,096 particles, 2.7 waves per SIMD, take 1.97 times as long as 7)."""
+    return max(1.0, 0.73 * lanes / (64.0 * simds))
 
 
-def use_parallel_sweep(npart: int, nblocks: int, simds: int) -> bool:
-    """The parallel sweep (about 7 times the lane work, in two launches)
-    against nblocks sequential launches."""
-    if nblocks < 2:
-        return False
-    seq = nblocks * _sweep_cost(npart, simds)
-    par = (_sweep_cost((6 * (nblocks - 1) + 1) * npart, simds)
-           + _sweep_cost((nblocks - 1) * npart, simds))
-    return par < 0.8 * seq
+def sweep_plan(npart: int, block_turns, simds: int) -> int:
+    """Sub-block length of the parallel sweep, in turns, or 0 for the
+    sequential one. The parallel sweep splits the checkpoint blocks (turn
+    counts block_turns) into S sub-blocks of g turns (their turn-start states
+    are all recorded) and makes two launches of g turns, one of
+    6 (S - 1) + 1 items and one of S - 1; the sequential sweep walks every
+    turn with npart lanes. Shorter sub-blocks mean more lanes and shorter
+    launches; the cost model picks g."""
+    turns = int(sum(block_turns))
+    if len(block_turns) < 1 or turns < 2:
+        return 0
+    seq = turns * _sweep_cost(npart, simds)
+    best, best_cost = 0, 0.8 * seq
+    g = 1
+    while g <= max(block_turns):
+        nsub = sum(-(-nt // g) for nt in block_turns)
+        if nsub >= 2:
+            cost = g * (_sweep_cost((6 * (nsub - 1) + 1) * npart, simds)
+                        + _sweep_cost((nsub - 1) * npart, simds))
+            if cost < best_cost:
+                best, best_cost = g, cost
+        g *= 2
+    return best
+
+
+def use_parallel_sweep(npart: int, nblocks: int, simds: int, checkpoint: int = 8) -> bool:
+    """Whether the cost model prefers the parallel sweep for nblocks
+    checkpoint blocks of `checkpoint` turns."""
+    return sweep_plan(npart, [checkpoint] * nblocks, simds) > 0
 
 
 def _device_simds(dev) -> int:
@@ -112,12 +137,28 @@ def _sweep_sequential(problem, blocks, states, lam, grad, n, stream):
                            stream=stream)
 
 
-def _sweep_parallel(problem, blocks, states, lam, grad, n, stream):
-    """Blocks in parallel: unit seeds through blocks 1..B-1 (with block
-    B-1's own sweep in the same launch), lambda at the block boundaries by
-    6x6 products, then blocks 0..B-2 in one launch."""
+def _sub_blocks(blocks, states, g: int):
+    """Sub-blocks of at most g turns of the checkpoint blocks: (turn-start
+    states of the sub-block and its end state, a contiguous view; turn
+    number of its first turn; its turn count), in turn order."""
+    subs = []
+    for k, (t0, nt, _) in enumerate(blocks):
+        for j in range(0, nt, g):
+            m = min(g, nt - j)
+            subs.append((t0 + j, m, states[k][j:j + m + 1]))
+    return subs
+
+
+def _sweep_parallel(problem, blocks, states, lam, grad, n, stream, g: int = 0):
+    """Sub-blocks of g turns (whole blocks for g = 0) in parallel: unit
+    seeds through sub-blocks 1..S-1 (with the last one's own sweep in the
+    same launch), lambda at the boundaries by 6x6 products, then sub-blocks
+    0..S-2 in one launch."""
     import torch
 
+    if g > 0:
+        blocks = _sub_blocks(blocks, states, g)
+        states = [st for _, _, st in blocks]
     nb = len(blocks)
     npar = len(problem.params)
     # seeds[k - 1, i]: adjoint of the end of block k seeded with e_i; after
@@ -212,11 +253,15 @@ class TrackingLoss:
                                        rest_energy=problem.rest, losses=False, stream=stream)
                 for k, (_, nt, _) in enumerate(blocks):
                     states[k][nt].copy_(cur[k])
-                par = problem.parallel
-                if par is None:
-                    par = use_parallel_sweep(n, len(blocks), _device_simds(particles.device))
-                if par and len(blocks) > 1:
-                    _sweep_parallel(problem, blocks, states, lam, grad, n, stream)
+                g = problem.sub_block
+                if g is None:
+                    g = sweep_plan(n, [nt for _, nt, _ in blocks], _device_simds(particles.device))
+                    if problem.parallel is True and g == 0:
+                        g = problem.checkpoint
+                    elif problem.parallel is False:
+                        g = 0
+                if g > 0 and sum(-(-nt // g) for _, nt, _ in blocks) > 1:
+                    _sweep_parallel(problem, blocks, states, lam, grad, n, stream, g)
                 else:
                     _sweep_sequential(problem, blocks, states, lam, grad, n, stream)
                 return grad.to(device=ctx.theta_device, dtype=torch.float64)

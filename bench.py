@@ -336,7 +336,7 @@ def bench_adjoint(args) -> dict:
     import torch
 
     from at_amd import lattice as L
-    from at_amd.adjoint import Problem, TrackingLoss, use_parallel_sweep
+    from at_amd.adjoint import Problem, TrackingLoss, sweep_plan
     from at_amd.workloads import gaussian_beam, large_ring
 
     ring = large_ring(radiation=False, cavity=True)
@@ -376,10 +376,11 @@ def bench_adjoint(args) -> dict:
     wall = time.perf_counter() - t0
     ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     fms = float(np.mean([a.elapsed_time(b) for a, b in fw]))
-    sweep = "parallel" if use_parallel_sweep(n, (T + 7) // 8, 4 * torch.cuda.get_device_properties(
-        dev).multi_processor_count) else "sequential"
+    g = sweep_plan(n, [8] * ((T + 7) // 8), 4 * torch.cuda.get_device_properties(
+        dev).multi_processor_count)
+    sweep = f"parallel, {g}-turn sub-blocks" if g else "sequential"
     seq_ms = None
-    if not args.no_compare and sweep == "parallel":
+    if not args.no_compare and g:
         # the same gradient with the blocks swept one after the other
         prob.parallel = False
         one()

@@ -162,17 +162,19 @@ def test_adjoint_batch_argument_errors():
 
 
 def test_parallel_adjoint_sweep_cost_model():
-    """The block-parallel sweep is chosen only while the beam leaves the
-    device's SIMDs idle."""
-    from at_amd.adjoint import use_parallel_sweep
+    """The parallel sweep is chosen only while the beam leaves the device's
+    SIMDs idle, with sub-blocks the shorter the fewer particles there are."""
+    from at_amd.adjoint import sweep_plan, use_parallel_sweep
 
     simds = 1024
-    assert use_parallel_sweep(4096, 8, simds)        # 64 waves on 1,024 SIMDs
-    assert use_parallel_sweep(512, 22, simds)
-    assert not use_parallel_sweep(4096, 1, simds)    # one block: nothing to split
-    assert not use_parallel_sweep(4096, 2, simds)    # two launches either way
-    assert not use_parallel_sweep(65536, 8, simds)   # one wave per SIMD already
-    assert not use_parallel_sweep(262144, 125, simds)
+    assert sweep_plan(4096, [8] * 8, simds) == 2      # 64 waves on 1,024 SIMDs
+    assert sweep_plan(128, [8] * 8, simds) == 1
+    assert sweep_plan(4096, [8], simds) == 1          # one block still splits into turns
+    assert sweep_plan(4096, [1], simds) == 0          # one turn: nothing to split
+    assert sweep_plan(65536, [8] * 8, simds) == 0     # one wave per SIMD already
+    assert sweep_plan(262144, [8] * 125, simds) == 0
+    assert use_parallel_sweep(4096, 8, simds)
+    assert not use_parallel_sweep(32768, 8, simds)
 
 
 def test_device_count_without_gpu_is_zero_or_more():

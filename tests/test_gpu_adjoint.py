@@ -148,7 +148,7 @@ def test_tangent_maps_equal_fd_of_oracle(name):
         assert err <= 2e-8 * scale, rows[ip] - jac
 
 
-def _ring_problem(checkpoint=8, parallel=None):
+def _ring_problem(checkpoint=8, parallel=None, sub_block=None):
     ring = large_ring(radiation=False, cavity=True)
     params = []
     for i, e in enumerate(ring):
@@ -156,7 +156,8 @@ def _ring_problem(checkpoint=8, parallel=None):
             params.append((i, 1))
         elif e.PassMethod.startswith("StrMPole") and e.MaxOrder == 2:
             params.append((i, 2))
-    return ring, Problem(ring, params, checkpoint=checkpoint, parallel=parallel)
+    return ring, Problem(ring, params, checkpoint=checkpoint, parallel=parallel,
+                         sub_block=sub_block)
 
 
 def _loss_cpu(ring, beam, nturns, edits, impl):
@@ -310,17 +311,21 @@ def test_adjoint_batch_items_equal_single_calls():
     assert torch.equal(lam_f, lam_d_ref)
 
 
-@pytest.mark.parametrize("checkpoint", [8, 5])
-def test_parallel_sweep_equals_sequential(checkpoint):
+@pytest.mark.parametrize("checkpoint,sub_block", [(8, None), (8, 1), (8, 3), (8, 8), (5, None)])
+def test_parallel_sweep_equals_sequential(checkpoint, sub_block):
+    """The parallel sweep over sub-blocks of 1, 2 (the cost model's choice
+    here), 3 (uneven: 3+3+2 per block) and 8 turns, and over blocks of 5
+    turns, equals the sequential sweep."""
     ring, pseq = _ring_problem(checkpoint=checkpoint, parallel=False)
-    _, ppar = _ring_problem(checkpoint=checkpoint, parallel=True)
+    _, ppar = _ring_problem(checkpoint=checkpoint, parallel=True, sub_block=sub_block)
     beam = gaussian_beam(4096, seed=8)
     nturns = 64
     l_seq, g_seq = loss_and_grad(pseq, beam, nturns)
     l_par, g_par = loss_and_grad(ppar, beam, nturns)
     gmax = np.abs(g_seq).max()
     err = np.abs(g_par - g_seq).max()
-    print(f"checkpoint {checkpoint}: loss {l_seq!r}, max |g_par - g_seq| = {err:.3e} "
+    print(f"checkpoint {checkpoint}, sub-block {sub_block}: loss {l_seq!r}, "
+          f"max |g_par - g_seq| = {err:.3e} "
           f"of max |g| {gmax:.3e}", flush=True)
     assert l_par == l_seq
     assert err <= 1e-11 * gmax
