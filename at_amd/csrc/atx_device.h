@@ -98,9 +98,11 @@ struct DevElem {
   int32_t t1, r1;     // pool offsets (valid when the flag is set)
   int32_t r2, t2;
   int32_t rap, eap;   // RApertures (4) / Limits (4); EApertures (2)
-  int32_t edge;       // EdgeMode at entrance | (EdgeMode at exit << 8)
+  int32_t edge;       // EdgeMode at entrance | (EdgeMode at exit << 8); DriftPass: low
+                      // word of the Length bits
   int32_t ops;        // pool offset of the slice op table (integrators other than 4), -1: none
-  int32_t fringe;     // QuadFringe at entrance | (QuadFringe at exit << 8)
+  int32_t fringe;     // QuadFringe at entrance | (QuadFringe at exit << 8); DriftPass: high
+                      // word of the Length bits
   int32_t fq;         // pool offset of the elegant fringe matrices: in-P, in-M, out-M, out-P (8 each)
 };
 

@@ -333,6 +333,14 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
     case K_DRIFT:
     case K_EXDRIFT:  // DriftPass.c:50-80, ExactDriftPass.c:46-75
       sc[S_D_LEN] = rd.dbl("Length");
+      if (e.kind == K_DRIFT) {
+        // the tracking kernel reads a drift's length from its header (edge,
+        // fringe: unused by drifts), saving a dependent load per drift
+        uint64_t bits;
+        std::memcpy(&bits, &sc[S_D_LEN], sizeof(bits));
+        h.edge = (int32_t)(uint32_t)bits;
+        h.fringe = (int32_t)(uint32_t)(bits >> 32);
+      }
       transforms(rd, h, pool);
       break;
     case K_CAVITY: {  // CavityPass.c:62-88
@@ -560,7 +568,7 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
   if (h.kind == K_CAVITY || h.kind == K_EXDRIFT || h.kind == K_EXMPOLE || h.kind == K_EXMPOLERAD)
     aerr = "pass method without an adjoint";
   if (h.flags & F_SCALE) aerr = "FieldScaling != 1";
-  if (h.fringe) aerr = "quadrupole fringe";
+  if (h.kind != K_DRIFT && h.fringe) aerr = "quadrupole fringe";  // a drift's holds its length
   if (h.ops >= 0) aerr = "integrator other than 4";
   for (int side = 0; side < 2; side++) {
     const int m = (h.edge >> (8 * side)) & 0xff;

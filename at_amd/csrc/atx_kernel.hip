@@ -1152,6 +1152,15 @@ __global__ void __launch_bounds__(64) diffusion_kernel(DiffArgs args) {
 // the register demand of the common passes only.
 template <bool EXT>
 __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int nturn) {
+  if (h.kind == K_DRIFT) {  // DriftPass.c:16-48; half the elements of a ring, tested first
+    const uint64_t bits = ((uint64_t)(uint32_t)h.fringe << 32) | (uint32_t)h.edge;
+    entry_xform(r, h, pool);
+    apertures(r, h, pool);
+    drift6(r, __builtin_bit_cast(double, bits));
+    apertures(r, h, pool);
+    exit_xform(r, h, pool);
+    return;
+  }
   if (EXT) {
     switch (h.kind) {
       case K_CAVITY: r = cavity(r, pool + h.sc, (h.flags & F_THICK) != 0); return;
@@ -1171,13 +1180,6 @@ __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int n
   switch (h.kind) {
     case K_IDENTITY:  // IdentityPass.c:20-43
       entry_xform(r, h, pool);
-      apertures(r, h, pool);
-      exit_xform(r, h, pool);
-      break;
-    case K_DRIFT:  // DriftPass.c:16-48
-      entry_xform(r, h, pool);
-      apertures(r, h, pool);
-      drift6(r, pool[h.sc + S_D_LEN]);
       apertures(r, h, pool);
       exit_xform(r, h, pool);
       break;
