@@ -676,22 +676,27 @@ constexpr int kBlock = 256;
 struct Staging {
   char* p = nullptr;
   size_t cap = 0;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr;  // recorded on streams of `device` only
+  int device = 0;
 };
 std::mutex g_stage_mu;
 std::vector<std::unique_ptr<Staging>> g_stage;
 
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream) {
   if (bytes == 0) return hipSuccess;
+  int dev = 0;
+  hipError_t de = hipGetDevice(&dev);
+  if (de != hipSuccess) return de;
   std::lock_guard<std::mutex> g(g_stage_mu);
   Staging* st = nullptr;
   for (auto& x : g_stage)
-    if (x->cap >= bytes && hipEventQuery(x->ev) == hipSuccess) {
+    if (x->device == dev && x->cap >= bytes && hipEventQuery(x->ev) == hipSuccess) {
       st = x.get();
       break;
     }
   if (!st) {
     auto n = std::make_unique<Staging>();
+    n->device = dev;
     n->cap = std::max(bytes, (size_t)(64 << 10));
     hipError_t e = hipHostMalloc((void**)&n->p, n->cap, hipHostMallocDefault);
     if (e != hipSuccess) return e;
