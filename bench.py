@@ -46,6 +46,8 @@ def measured_traffic(particles: int, turns: int):
     if t.get("particles") != particles or t.get("turns_per_launch") != turns:
         return None
     return t.get("bytes_per_launch")
+
+
 METRIC = "particle-element passes/s FP64 + %FP64 peak; served p50/p99 latency vs plain"
 
 
@@ -463,8 +465,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              f"(particles shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (particles "
+              f"and lattice instances shard across GPUs with no exchange during tracking; "
+              f"the bench measures one GPU's share)", file=sys.stderr)
         return 2
 
     import numpy as np
