@@ -24,7 +24,7 @@ extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int,
                                             hipStream_t);
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
 extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
-extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int32_t*, hipStream_t);
+extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int, int32_t*, hipStream_t);
 extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, int32_t, hipStream_t);
 
 namespace atx {
@@ -808,11 +808,13 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   reap_graphs();
   const size_t ninst = insts.size();
   int nblocks = 0;
+  int nblocks_fine = 0;  // grid of the later blocks: room for 64 particles per workgroup
   int maxt = 0;
   int64_t nidx = 0;
   for (auto& in : insts) {
     in.block_begin = nblocks;
     nblocks += (int)((in.npart + block - 1) / block);
+    nblocks_fine += (int)((in.npart + 63) / 64);
     maxt = std::max(maxt, (int)in.nturns);
     nidx += in.npart;
   }
@@ -842,7 +844,13 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   const size_t nlive_off = bytes;
   bytes += ninst * sizeof(int32_t);
   const size_t live_off = bytes;  // live workgroup map of the current block
-  bytes += (ninst + 1) * sizeof(int32_t);
+  bytes += (ninst + 2) * sizeof(int32_t);
+  // survivors that fit in two waves per SIMD run one wave per workgroup
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 0;
+  const int fine_limit = 2 * 64 * 4 * ncu;
   char* d = nullptr;
   HIPCHK(hipMallocAsync((void**)&d, bytes, stream));
   std::vector<char> host(upload);
@@ -879,11 +887,12 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
     for (int k = 0; k < nblk && le == hipSuccess; k++) {
       const Instance* t = (const Instance*)(d + (size_t)k * tab);
       // after the first block the survivors' workgroups lead the grid
-      if (k > 0) le = atx_launch_live_map(t, (int)ninst, block, live, cap);
+      if (k > 0) le = atx_launch_live_map(t, (int)ninst, block, fine_limit, live, cap);
       if (le != hipSuccess) break;
       const int32_t* lb = k > 0 ? live : nullptr;
-      le = strict ? atx_launch_track_strict(t, (int)ninst, nblocks, block, ext ? 1 : 0, lb, cap)
-                  : atx_launch_track_fast(t, (int)ninst, nblocks, block, ext ? 1 : 0, lb, cap);
+      const int grid = k > 0 ? nblocks_fine : nblocks;
+      le = strict ? atx_launch_track_strict(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap)
+                  : atx_launch_track_fast(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap);
       if (le == hipSuccess && k + 1 < nblk)
         le = atx_launch_repack((const Instance*)(d + (size_t)(k + 1) * tab), (int)ninst, k == 0 ? 1 : 0, cap);
     }

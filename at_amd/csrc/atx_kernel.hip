@@ -1233,9 +1233,11 @@ __device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
 
 // live_begin (repacked turn blocks after the first; NULL otherwise): first
 // workgroup of each instance among the workgroups that hold survivors, with
-// live_begin[ninst] their total, so that the live workgroups of all
-// instances are the leading ones of the grid and the hardware spreads them
-// evenly over the CUs; the other workgroups end at once.
+// live_begin[ninst] their total and live_begin[ninst + 1] the particles per
+// workgroup (the block size, or 64 when few survive), so that the live
+// workgroups of all instances are the leading ones of the grid and the
+// hardware spreads them evenly over the CUs; the other workgroups end at
+// once.
 template <bool EXT>
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst,
                                                     const int32_t* __restrict__ live_begin) {
@@ -1245,8 +1247,10 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   int lo = 0, hi = ninst - 1;
   const int bid = (int)blockIdx.x;
   int wbegin;
+  int B = (int)blockDim.x;  // particle slots per workgroup
   if (live_begin) {
     if (bid >= live_begin[ninst]) return;
+    B = live_begin[ninst + 1];
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (live_begin[mid] <= bid) lo = mid; else hi = mid - 1;
@@ -1261,7 +1265,6 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   }
   cInstance& I = insts[lo];
   const int tid = (int)threadIdx.x;
-  const int B = (int)blockDim.x;
   const int npart = I.npart;
   // lanes of a repacked launch take the surviving particles idx[0..nlive)
   const int32_t* idx = I.idx;
@@ -1467,27 +1470,47 @@ __global__ void __launch_bounds__(1024) repack_kernel(const Instance* __restrict
 }  // namespace atx
 
 namespace atx {
+// Live particles of instance i with turns in the block (0 for the others).
+__device__ __forceinline__ int live_count(const Instance& I) {
+  return I.nturns > 0 ? (I.idx ? *I.nlive : I.npart) : 0;
+}
+
 // Map of the live workgroups for the next turn block (one workgroup):
-// live_begin[i] = sum over j < i of ceil(nlive_j / block) for the instances
-// with turns in the block (none for the others), live_begin[ninst] = total.
-// Chunks of 1,024 instances: inclusive scan in each wave, then the wave
-// totals through LDS, carrying the running sum from chunk to chunk.
+// live_begin[i] = sum over j < i of ceil(nlive_j / g), live_begin[ninst] =
+// the total and live_begin[ninst + 1] = g, the particles per workgroup: the
+// block size, or 64 when all survivors fit in fine_limit lanes (then each
+// workgroup runs one wave and the dispatcher spreads single waves over the
+// SIMDs, which a tail of few survivors needs; the grid holds enough
+// workgroups for either). Chunks of 1,024 instances: a wave-level scan, then
+// the wave totals through LDS, carrying the running sum from chunk to chunk.
 __global__ void __launch_bounds__(1024) live_map_kernel(const Instance* __restrict__ insts, int ninst,
-                                                        int block, int32_t* __restrict__ live_begin) {
+                                                        int block, int fine_limit,
+                                                        int32_t* __restrict__ live_begin) {
   __shared__ int s_wave[16];
+  __shared__ int s_total;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwaves = (int)blockDim.x >> 6;
+  // total survivors
+  int part = 0;
+  for (int i = tid; i < ninst; i += (int)blockDim.x) part += live_count(insts[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if (lane == 0) s_wave[wave] = part;
+  __syncthreads();
+  if (tid == 0) {
+    int t = 0;
+    for (int k = 0; k < nwaves; k++) t += s_wave[k];
+    s_total = t;
+  }
+  __syncthreads();
+  const int g = s_total <= fine_limit ? 64 : block;
   int carry = 0;
   for (int base = 0; base < ninst; base += (int)blockDim.x) {
     const int i = base + tid;
     int c = 0;
-    if (i < ninst) {
-      const Instance& I = insts[i];
-      const int n = I.idx ? *I.nlive : I.npart;
-      c = I.nturns > 0 ? (n + block - 1) / block : 0;
-    }
+    if (i < ninst) c = (live_count(insts[i]) + g - 1) / g;
     int v = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -1506,7 +1529,10 @@ __global__ void __launch_bounds__(1024) live_map_kernel(const Instance* __restri
     carry += total;
     __syncthreads();
   }
-  if (tid == 0) live_begin[ninst] = carry;
+  if (tid == 0) {
+    live_begin[ninst] = carry;
+    live_begin[ninst + 1] = g;
+  }
 }
 }  // namespace atx
 
@@ -1518,10 +1544,10 @@ extern "C" hipError_t atx_launch_repack(const atx::Instance* d_insts, int ninst,
 }
 
 extern "C" hipError_t atx_launch_live_map(const atx::Instance* d_insts, int ninst, int block,
-                                          int32_t* live_begin, hipStream_t stream) {
+                                          int fine_limit, int32_t* live_begin, hipStream_t stream) {
   if (ninst <= 0) return hipSuccess;
   hipLaunchKernelGGL(atx::live_map_kernel, dim3(1), dim3(1024), 0, stream, d_insts, ninst, block,
-                     live_begin);
+                     fine_limit, live_begin);
   return hipGetLastError();
 }
 
