@@ -960,16 +960,7 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   inst.block_begin = 0;
   inst.losses = losses ? 1 : 0;
   inst.elempass = elempass ? 1 : 0;
-  // A beam of at most one wave per SIMD runs in 64-thread workgroups: its
-  // waves then spread over four times as many CUs (3 % faster for 4,096
-  // particles on the large ring than 256-thread workgroups).
-  int block = kBlock;
-  int dev = 0, ncu = 0;
-  if (hipGetDevice(&dev) == hipSuccess &&
-      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-      (int64_t)np <= (int64_t)256 * ncu)
-    block = 64;
-  return launch_instances(insts, {&slots[0]}, block, stream, strict, lat->has_ext);
+  return launch_instances(insts, {&slots[0]}, kBlock, stream, strict, lat->has_ext);
 }
 
 int check_partial(int nelem, int nturns, int& start, int& end) {
