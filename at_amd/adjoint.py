@@ -85,10 +85,11 @@ class Problem:
 def _sweep_cost(lanes: int, simds: int) -> float:
     """Time per turn of one adjoint launch over `lanes` particles, in units
     of a launch with at most one wave per SIMD: below that the dependent FP64
-    chain of each lane sets the time; above it each further wave per SIMD
-    adds about 0.73 of a wave's time (measured on the large ring: 43 blocks
-    of 4,096 particles, 2.7 waves per SIMD, take 1.97 times as long as 7)."""
-    return max(1.0, 0.73 * lanes / (64.0 * simds))
+    chain of each lane sets the time; above it the time grows as
+    0.536 + 0.533 w with w waves per SIMD (fitted to launches over the large
+    ring of 4,096-particle items: 1.57 at w = 1.9, 1.97 at w = 2.7, 6.77 at
+    w = 11.7; it orders 1-, 2- and 4-turn sub-blocks as measured)."""
+    return max(1.0, 0.536 + 0.533 * lanes / (64.0 * simds))
 
 
 def sweep_plan(npart: int, block_turns, simds: int) -> int:

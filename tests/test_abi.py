@@ -167,7 +167,9 @@ def test_parallel_adjoint_sweep_cost_model():
     from at_amd.adjoint import sweep_plan, use_parallel_sweep
 
     simds = 1024
-    assert sweep_plan(4096, [8] * 8, simds) == 2      # 64 waves on 1,024 SIMDs
+    assert sweep_plan(4096, [8] * 8, simds) == 1      # 64 waves on 1,024 SIMDs
+    assert sweep_plan(8192, [8] * 8, simds) == 1
+    assert sweep_plan(16384, [8] * 8, simds) == 0
     assert sweep_plan(128, [8] * 8, simds) == 1
     assert sweep_plan(4096, [8], simds) == 1          # one block still splits into turns
     assert sweep_plan(4096, [1], simds) == 0          # one turn: nothing to split
