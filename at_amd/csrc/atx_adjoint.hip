@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "atx_device.h"
+#include "atx_libm.h"
 
 namespace atx {
 namespace adj {
@@ -227,9 +228,14 @@ __device__ __forceinline__ void rf_rev(V6& r, V6& l, cDevElem& h, cdouble* pool,
   if (h.flags & F_THICK) drift_rev(r, l, s[S_RF_HALF]);
   if (h.flags & F_RFKICK) {
     const double tw = s[S_RF_TWOPIF];
-    const double ph = tw * ((r.ct - s[S_RF_LAG]) / 2.99792458e8 - s[S_RF_HFT0] * nt) - s[S_RF_PHILAG];
-    double sn, cs;
-    sincos(ph, &sn, &cs);
+    double ph, sn;
+    {
+      // the forward kick's phase and sine, bit for bit, so the inverse is exact
+#pragma clang fp contract(off)
+      ph = tw * ((r.ct - s[S_RF_LAG]) / 2.99792458e8 - s[S_RF_HFT0] * nt) - s[S_RF_PHILAG];
+      sn = libm::host_sin(ph);
+    }
+    const double cs = cos(ph);
     r.dp += s[S_RF_NV] * sn;  // undo dp += -nv sin(ph)
     l.ct += -s[S_RF_NV] * cs * tw / 2.99792458e8 * l.dp;
   }

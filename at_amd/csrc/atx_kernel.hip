@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "atx_device.h"
+#include "atx_libm.h"
 
 #ifndef ATX_STRICT
 #error "ATX_STRICT must be defined to 0 or 1"
@@ -656,10 +657,12 @@ __device__ __forceinline__ void thin(R6& r, cDevElem& h, cdouble* pool) {
   if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
 }
 
-// attrackfunc.c:5-42: hft = h/f - T0 prepared on the host.
+// attrackfunc.c:5-42: hft = h/f - T0 prepared on the host. The sine is the
+// host C library's, restated bit for bit (atx_libm.h), in both builds.
 __device__ __noinline__ double rf_kick(double ct, double nv, double twopif, double lag,
                                        double hft0, double philag, double nt) {
-  return -nv * sin(twopif * ((ct - lag) / 2.99792458e8 - hft0 * nt) - philag);
+#pragma clang fp contract(off)
+  return -nv * libm::host_sin(twopif * ((ct - lag) / 2.99792458e8 - hft0 * nt) - philag);
 }
 
 __device__ __forceinline__ void rf_cavity(R6& r, cDevElem& h, cdouble* pool, int nturn) {
@@ -679,7 +682,8 @@ __device__ __forceinline__ void rf_cavity(R6& r, cDevElem& h, cdouble* pool, int
 // CavityPass.c:18-56: no turn dependence (nv = V/E prepared on the host).
 __device__ __noinline__ R6 cavity(R6 r, cdouble* s, bool thick) {
   if (thick) drift6(r, s[S_RF_HALF]);
-  r.dp += -s[S_RF_NV] * sin(s[S_RF_TWOPIF] * (r.ct - s[S_RF_LAG]) / 2.99792458e8 - s[S_RF_PHILAG]);
+  r.dp += -s[S_RF_NV] *
+          libm::host_sin(s[S_RF_TWOPIF] * (r.ct - s[S_RF_LAG]) / 2.99792458e8 - s[S_RF_PHILAG]);
   if (thick) drift6(r, s[S_RF_HALF]);
   return r;
 }

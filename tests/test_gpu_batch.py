@@ -2,8 +2,7 @@
 
 Each instance of a batch must give exactly what the same instance gives
 alone (same kernel arithmetic, bit-exact in both builds) and, in the strict
-build, what the oracle gives (bit-exact on these lattices without an RF
-cavity; the RF sin is the only device transcendental)."""
+build, what the oracle gives (bit-exact, RF cavity included)."""
 
 import numpy as np
 import pytest
@@ -17,7 +16,7 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 
-def _instances(n_inst=3, ncells=2, cavity=False):
+def _instances(n_inst=3, ncells=2, cavity=True):
     base = large_ring(ncells=ncells, cavity=cavity)
     sizes = [300, 517, 64][:n_inst]
     lats = [error_realisation(base, 100 + i) for i in range(n_inst)]

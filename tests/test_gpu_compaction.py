@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _ring():
-    # no cavity: the strict build then matches the oracle bit for bit
-    return error_realisation(large_ring(ncells=2, cavity=False), 11)
+    # the strict build matches the oracle bit for bit, RF cavity included
+    return error_realisation(large_ring(ncells=2, cavity=True), 11)
 
 
 def _grid(n=3001):

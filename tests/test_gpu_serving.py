@@ -84,9 +84,8 @@ def test_one_by_one_baseline_equals_served():
 
 
 def test_strict_served_request_equals_oracle():
-    # without the cavity (whose sin is the only device transcendental) the
-    # strict build and the oracle agree bit for bit
-    lat = error_realisation(large_ring(ncells=2, cavity=False), 77)
+    # the strict build and the oracle agree bit for bit, RF cavity included
+    lat = error_realisation(large_ring(ncells=2, cavity=True), 77)
     energy, rest = lat.energy, lat.particle.rest_energy
     dl = engine.DeviceLattice(lat)
     q = Request(0, 0, da_polar_grid(500, xmax=12e-3, ymax=5e-3), 9)
