@@ -68,12 +68,18 @@ class Problem:
             self.slots[e] = i
             self.coeffs[e] = k
         self._pb = {e: np.array(lattice[e].PolynomB, dtype=np.float64) for e, _ in self.params}
+        self._current = self.initial_strengths()  # strengths on the device lattice
 
     def initial_strengths(self) -> np.ndarray:
         return np.array([self._pb[e][k] for e, k in self.params])
 
     def set_strengths(self, values) -> None:
-        vals = np.asarray(values, dtype=np.float64)
+        """Put these strengths on the device lattice (recompiled on its next
+        launch); nothing happens when they are already there."""
+        vals = np.array(values, dtype=np.float64)
+        if vals.shape == self._current.shape and np.array_equal(vals, self._current):
+            return
+        self._current = vals
         touched = {}
         for (e, k), v in zip(self.params, vals):
             pb = touched.setdefault(e, self._pb[e].copy())
@@ -210,7 +216,8 @@ class TrackingLoss:
         class _F(torch.autograd.Function):
             @staticmethod
             def forward(ctx, theta):
-                problem.set_strengths(theta.detach().cpu().numpy())
+                strengths_np = theta.detach().cpu().numpy().copy()
+                problem.set_strengths(strengths_np)
                 r = particles.detach().clone()
                 n = r.shape[0]
                 cp = problem.checkpoint
@@ -232,6 +239,7 @@ class TrackingLoss:
                     t += nt
                 loss, lam = _loss_and_seed(r)
                 ctx.ckpts = ckpts
+                ctx.strengths = strengths_np
                 ctx.lam = lam
                 ctx.theta_device = theta.device
                 ctx.final = r
@@ -247,6 +255,9 @@ class TrackingLoss:
                 blocks = ctx.ckpts
                 if not blocks:
                     return grad.to(device=ctx.theta_device, dtype=torch.float64)
+                # the lattice is shared: another forward may have set other
+                # strengths since this one; recompute and sweep with this one's
+                problem.set_strengths(ctx.strengths)
                 states = [torch.empty((nt + 1, n, 6), dtype=torch.float64,
                                       device=particles.device) for _, nt, _ in blocks]
                 cur = [ck.clone() for _, _, ck in blocks]

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -923,21 +924,17 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   return ATX_OK;
 }
 
-// Device-side core of atx_track: all pointers are device pointers.
-int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
-                 int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
-                 double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
-                 int losses, bool elempass, hipStream_t stream, bool strict) {
+// The instance of one atx_track call and its refpt slot table (first turn,
+// then later turns), with the argument checks that need them.
+int single_instance(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
+                    int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
+                    double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
+                    int losses, bool elempass, int32_t* slots, Instance& inst) {
   const int nelem = (int)lat->elems.size();
-  std::vector<std::vector<int32_t>> slots(1, std::vector<int32_t>(2 * (size_t)(nelem + 1)));
-  const int n_first = turn_slots(refpts, (int)nref, start, end, nelem, slots[0].data());
-  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, slots[0].data() + nelem + 1);
+  const int n_first = turn_slots(refpts, (int)nref, start, end, nelem, slots);
+  const int n_std = turn_slots(refpts, (int)nref, 0, end, nelem, slots + nelem + 1);
   if ((n_first > 0 || n_std > 0) && !d_rout) return fail(ATX_E_VALUE, "r_out is NULL but refpts request copies");
   if (losses && (!d_lf || !d_lt || !d_le || !d_lc)) return fail(ATX_E_VALUE, "loss arrays are NULL");
-  if (np == 0 || nturns <= 0) return ATX_OK;
-
-  std::vector<Instance> insts(1);
-  Instance& inst = insts[0];
   std::memset(&inst, 0, sizeof(inst));
   inst.elems = lat->d_elems;
   inst.pool = lat->d_pool;
@@ -960,6 +957,21 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   inst.block_begin = 0;
   inst.losses = losses ? 1 : 0;
   inst.elempass = elempass ? 1 : 0;
+  return ATX_OK;
+}
+
+// Device-side core of atx_track: all pointers are device pointers.
+int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
+                 int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
+                 double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
+                 int losses, bool elempass, hipStream_t stream, bool strict) {
+  const int nelem = (int)lat->elems.size();
+  std::vector<std::vector<int32_t>> slots(1, std::vector<int32_t>(2 * (size_t)(nelem + 1)));
+  std::vector<Instance> insts(1);
+  int rc = single_instance(lat, d_r, pstride, cstride, np, nturns, refpts, nref, turn0, start, end,
+                           d_rout, d_lf, d_lt, d_le, d_lc, losses, elempass, slots[0].data(), insts[0]);
+  if (rc) return rc;
+  if (np == 0 || nturns <= 0) return ATX_OK;
   return launch_instances(insts, {&slots[0]}, kBlock, stream, strict, lat->has_ext);
 }
 
@@ -1175,6 +1187,34 @@ int atx_track(atx_lattice* lat, double* r_in, uint32_t np, int32_t nturns, const
   return ATX_OK;
 }
 
+int atx_shutdown(void) {
+  g_err.clear();
+  hipError_t first = hipSuccess;
+  auto keep = [&](hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  };
+  {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    for (auto& pg : g_graphs) {
+      keep(hipEventSynchronize(pg.done));
+      keep(hipGraphExecDestroy(pg.exec));
+      keep(hipEventDestroy(pg.done));
+    }
+    g_graphs.clear();
+  }
+  {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    for (auto& st : g_stage) {
+      keep(hipEventSynchronize(st->ev));
+      keep(hipEventDestroy(st->ev));
+      keep(hipHostFree(st->p));
+    }
+    g_stage.clear();
+  }
+  if (first != hipSuccess) return fail(ATX_E_RUNTIME, std::string("atx_shutdown: ") + hipGetErrorString(first));
+  return ATX_OK;
+}
+
 int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns, int32_t turn,
                     double energy, double rest_energy, int32_t losses, int32_t block_size,
                     void* stream_, uint32_t flags) {
@@ -1191,16 +1231,25 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
   bool ext = false;
   std::map<int, std::vector<int32_t>> empty_slots;  // refpts = [] table per ring length
   std::vector<const std::vector<int32_t>*> slots;
-  std::vector<std::unique_lock<std::mutex>> locks;
   for (int i = 0; i < nitems; i++) {
     const atx_batch_item& it = items[i];
     if (!it.lattice) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL lattice");
     if (it.num_particles > 0 && !it.r) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL r");
     if (losses && it.num_particles > 0 && (!it.lost_flag || !it.lost_turn || !it.lost_elem || !it.lost_coord))
       return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": loss arrays are NULL");
-    bool seen = false;
-    for (int j = 0; j < i; j++) seen |= (items[j].lattice == it.lattice);
-    if (!seen) locks.emplace_back(it.lattice->mu);
+  }
+  // Each distinct lattice is locked once, in address order: two calls that
+  // list the same lattices in different orders take the locks in the same
+  // order and cannot deadlock.
+  std::vector<atx_lattice*> distinct;
+  for (int i = 0; i < nitems; i++) distinct.push_back(items[i].lattice);
+  std::sort(distinct.begin(), distinct.end(), std::less<atx_lattice*>());
+  distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  locks.reserve(distinct.size());
+  for (atx_lattice* l : distinct) locks.emplace_back(l->mu);
+  for (int i = 0; i < nitems; i++) {
+    const atx_batch_item& it = items[i];
     int rc = ensure_device_lattice(it.lattice, energy, rest_energy, false);
     if (rc) return fail(rc, "item " + std::to_string(i) + ": " + g_err);
     if (it.num_particles == 0) continue;

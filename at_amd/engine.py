@@ -8,6 +8,7 @@ only compute path: importing this module raises when it is missing.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
@@ -39,7 +40,7 @@ EXPORTS = (
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
     "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
-    "atx_diffusion_matrices", "atx_adjoint_batch",
+    "atx_diffusion_matrices", "atx_adjoint_batch", "atx_shutdown",
 )
 
 
@@ -130,9 +131,14 @@ def load_library(path: str | None = None) -> C.CDLL:
         lib.atx_device_info.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.POINTER(C.c_int), C.c_char_p, C.c_int]
         lib.atx_set_device.argtypes = [C.c_int]
+        lib.atx_shutdown.argtypes = []
         for name in EXPORTS:
             getattr(lib, name).restype = C.c_char_p if name == "atx_last_error" else C.c_int
         _lib = lib
+        # release the engine's graphs, events and pinned buffers while the
+        # HIP runtime is still whole: Python's exit handlers run before the
+        # shared libraries' destructors
+        atexit.register(lib.atx_shutdown)
         return lib
 
 

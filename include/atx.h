@@ -214,6 +214,16 @@ int atx_element_set_array_at(atx_lattice *lat, int32_t index, const char *name,
 /* Turn counter after the last call (at.c:656 last_turn). */
 int atx_lattice_turn_counter(const atx_lattice *lat, int64_t *turn);
 
+/* Release the HIP objects the engine keeps between calls: executable graphs
+ * of repacked launches not yet destroyed and their events (waited for
+ * first), and the pinned staging buffers of the launch tables with their
+ * events. Lattice handles are not touched. The engine stays usable: later
+ * calls create what they need again. For a process's end, before the HIP
+ * runtime unloads (the Python binding registers it with atexit). No
+ * counterpart in the reference, whose GPU backend keeps its objects in
+ * static caches until exit (atgpu/PyInterface.cpp:135-137). */
+int atx_shutdown(void);
+
 /* --- Devices -------------------------------------------------------------
  * Replaces at_gpuinfo (atgpu/PyInterface.cpp:110-130): one record per
  * device: name, architecture (gfx name), compute-unit count, platform. */

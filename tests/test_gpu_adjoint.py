@@ -228,6 +228,38 @@ def test_gradient_additive_over_particles_and_checkpoint_independent():
     assert np.abs(g3 - g8).max() <= 1e-11 * np.abs(g8).max()
 
 
+def test_backward_uses_its_own_forward_strengths():
+    """Two forwards at different strengths on one Problem (one device
+    lattice), then the backward of the first: its recompute and sweep run
+    with the first forward's strengths, so the gradient equals
+    loss_and_grad at those strengths."""
+    import torch
+
+    from at_amd.adjoint import TrackingLoss
+
+    ring, prob = _ring_problem(checkpoint=8)
+    beam = gaussian_beam(512, seed=9)
+    nturns = 16
+    theta_a = prob.initial_strengths()
+    theta_b = theta_a * (1.0 + 1e-3)
+    la_ref, ga_ref = loss_and_grad(prob, beam, nturns, strengths=theta_a)
+    parts = torch.from_numpy(np.ascontiguousarray(beam.T)).to(torch.device("cuda:0"))
+    ta = torch.tensor(theta_a, dtype=torch.float64, requires_grad=True)
+    tb = torch.tensor(theta_b, dtype=torch.float64, requires_grad=True)
+    loss_a = TrackingLoss.apply(prob, ta, parts, nturns)
+    loss_b = TrackingLoss.apply(prob, tb, parts, nturns)
+    loss_a.backward()
+    assert float(loss_a.detach().cpu()) == la_ref
+    assert float(loss_b.detach().cpu()) != la_ref
+    # the gradient sums per-wave atomics: equal to rounding (a backward at
+    # theta_b's strengths would be off by about 1e-3 of max|g|)
+    gmax = np.abs(ga_ref).max()
+    assert np.abs(ta.grad.numpy() - ga_ref).max() <= 1e-12 * gmax
+    loss_b.backward()
+    _, gb_ref = loss_and_grad(prob, beam, nturns, strengths=theta_b)
+    assert np.abs(tb.grad.numpy() - gb_ref).max() <= 1e-12 * np.abs(gb_ref).max()
+
+
 def test_adjoint_rejects_radiating_lattice():
     ring = large_ring(radiation=True, cavity=True, ncells=1)
     idx = next(i for i, e in enumerate(ring) if isinstance(e, L.Quadrupole))

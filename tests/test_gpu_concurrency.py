@@ -64,3 +64,55 @@ def test_threads_on_own_streams_equal_sequential():
         np.testing.assert_array_equal(out[k], ref[k], err_msg=f"thread {k}")
         nlost += int(np.isnan(ref[k][:, 0]).sum())
     print(f"{sum(b.shape[1] for b in beams)} particles, {nlost} lost", flush=True)
+
+
+def test_batches_listing_lattices_in_opposite_orders():
+    """Two threads batch the same two lattices, one as [A, B] and the other
+    as [B, A], many times over: the engine locks each call's lattices in one
+    global order, so neither waits for the other forever, and every launch
+    gives the results of the same launch made alone."""
+    import torch
+
+    ring = large_ring(ncells=1, cavity=True)
+    energy, rest = ring.energy, ring.particle.rest_energy
+    dev = torch.device("cuda:0")
+    dla, dlb = engine.DeviceLattice(ring), engine.DeviceLattice(ring)
+    beam = gaussian_beam(256, seed=8, scale=3.0)
+    host = np.ascontiguousarray(beam.T)
+
+    def run(order, stream=None):
+        rs = [torch.from_numpy(host).to(dev) for _ in order]
+        engine.track_batch([{"lattice": dl, "r": r.data_ptr(), "n": r.shape[0]}
+                            for dl, r in zip(order, rs)], 2, energy=energy, rest_energy=rest,
+                           losses=False, stream=stream)
+        return rs
+
+    ref = run([dla, dlb])[0]
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy()
+    errors, done = [], [0, 0]
+
+    def work(k, order):
+        try:
+            torch.cuda.set_device(dev)
+            st = torch.cuda.Stream(device=dev)
+            with torch.cuda.stream(st):
+                for _ in range(40):
+                    rs = run(order, stream=st.cuda_stream)
+                    st.synchronize()
+                    for r in rs:
+                        if not np.array_equal(r.cpu().numpy(), ref, equal_nan=True):
+                            raise AssertionError(f"thread {k}: results differ")
+                    done[k] += 1
+        except Exception as exc:  # noqa: BLE001 - reported below
+            errors.append(exc)
+
+    threads = [threading.Thread(target=work, args=(0, [dla, dlb]), daemon=True),
+               threading.Thread(target=work, args=(1, [dlb, dla]), daemon=True)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads), f"threads blocked after {done} launches"
+    assert not errors, errors
+    assert done == [40, 40]
