@@ -1,8 +1,8 @@
 """Derive the large-ring cell from the ESRF-EBS hybrid multi-bend achromat
-cell (the reference's own test fixture machine_data/hmba.mat) and freeze
-it as at_amd/data/large_ring_cell.json.
+cell (the reference's own test fixture machine_data/hmba.mat, kept as
+tests/golden/hmba.mat) and freeze it as at_amd/data/large_ring_cell.json.
 
-    python -m at_amd.data.make_large_ring /path/to/hmba.mat
+    python tests/golden/make_large_ring.py [path/to/hmba.mat]
 
 Transformations (the cell keeps the HMBA optics, so it is stable and
 chromatically corrected):
@@ -29,7 +29,8 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-OUT = os.path.join(HERE, "large_ring_cell.json")
+ROOT = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(ROOT, "at_amd", "data", "large_ring_cell.json")
 
 _GROUPS = [r"^DL1A_\d$", r"^DL2B_\d$", r"^DL2D_\d$", r"^DL1E_\d$", r"^(ml)?DQ1B$", r"^(ml)?DQ2C_\d$",
            r"^(ml)?DQ1D$"]
@@ -43,7 +44,7 @@ def _group(name: str):
 
 
 def convert(hmba_path: str) -> list[dict]:
-    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    sys.path.insert(0, ROOT)
     from at_amd.lattice import load_mat
 
     ring = load_mat(hmba_path)
@@ -96,7 +97,7 @@ def convert(hmba_path: str) -> list[dict]:
 
 
 def main(argv):
-    path = argv[1] if len(argv) > 1 else "/root/reference/machine_data/hmba.mat"
+    path = argv[1] if len(argv) > 1 else os.path.join(HERE, "hmba.mat")
     items = convert(path)
     lcell = 992 * 299792458.0 / 352.0e6 / 32
     total = sum(it["L"] for it in items)
