@@ -32,7 +32,7 @@ _C_LIGHT = 299792458.0
 _DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 # Large ring: 32 copies of the frozen cell in data/large_ring_cell.json
-# (derived from the ESRF-EBS HMBA cell by data/make_large_ring.py), radiating
+# (derived from the ESRF-EBS HMBA cell by tests/golden/make_large_ring.py), radiating
 # bends and quadrupoles, one RF cavity, one rectangular aperture per cell.
 LARGE_RING = {
     "ncells": 32,
