@@ -18,13 +18,10 @@ from .tracking import (  # noqa: E402,F401
 )
 from . import acceptance as _acceptance  # noqa: E402
 from . import physics as _physics  # noqa: E402
-from . import envelope as _envelope  # noqa: E402
 from .acceptance import (  # noqa: E402,F401
     GridMode, get_1d_acceptance, get_acceptance, get_horizontal_acceptance,
     get_momentum_acceptance, get_vertical_acceptance,
 )
-from .envelope import get_tunes_damp, ohmi_envelope  # noqa: E402,F401
-from .particles import beam, emittances_from_beam, sigma_matrix  # noqa: E402,F401
 from .physics import (  # noqa: E402,F401
     DConstant, ELossMethod, find_elem_m66, find_m44, find_m66, find_orbit, find_orbit4,
     find_orbit6, find_sync_orbit, get_energy_loss, get_timelag_fromU0,
@@ -37,5 +34,4 @@ for _name in ("find_orbit", "find_orbit4", "find_orbit6", "find_sync_orbit", "fi
 for _name in ("get_acceptance", "get_1d_acceptance", "get_horizontal_acceptance",
               "get_vertical_acceptance", "get_momentum_acceptance"):
     setattr(Lattice, _name, getattr(_acceptance, _name))
-Lattice.ohmi_envelope = _envelope.ohmi_envelope
 from .formats import load_json, load_lattice, load_repr, save_json, save_lattice, save_repr  # noqa: E402,F401

@@ -1,19 +1,43 @@
 """Dynamic-aperture (acceptance) driver on the engine.
 
-Restates pyat's boundary search (pyat/at/acceptance/boundary.py) and its
-front-ends (acceptance.py): particle grids in one or two planes (RADIAL,
-CARTESIAN), the survival mask of one multi-turn tracking launch
-(get_survived: losses=True, refpts=None, in place), the boundary of the
-surviving region, and the RECURSIVE radial search that tracks only the
-particles next to the boundary. Every tracking call goes to the engine.
-GridMode.FLOODFILL is not offered (the reference has it for CPU
+The caller-facing surface is pyat's (pyat/at/acceptance/acceptance.py:
+``get_acceptance``, ``get_1d_acceptance`` and its three plane shortcuts,
+``GridMode``), returning (boundary, survived, tracked). Two ways of finding
+the boundary:
+
+* **Whole grids** (RADIAL, CARTESIAN; boundary.py:112-148 define the point
+  sets): every point of every observation point's grid is moved to the
+  ring's entrance (one partial turn from the observation point) and all of
+  them are tracked in ONE launch; the survival mask is then read as an
+  image (angle x radius, or y x x) and its edge extracted with array
+  operations.
+* **Radial walk** (RECURSIVE, boundary.py:409-503): along each angle a walk
+  steps outwards; after a loss it steps back and continues at a finer step,
+  until a loss at the finest step. The walk's points and results are the
+  reference's (the boundary is quantised by the walk, and a point near the
+  boundary may be chaotic, so only the same points give the same answer,
+  bit for bit): the walk keeps each ray's position both as an integer
+  number of finest steps (the identity of a point, for the cache of points
+  already tracked) and as the floating-point coordinates the reference's
+  arithmetic gives along the same path (the particles actually tracked).
+  What is the engine's own is the scheduling: a round in which some ray
+  needs a new point tracks, in the same launch, up to ``lookahead`` points
+  that ray may need next — the walk played forward from the pending point
+  for both outcomes of every undecided point, nearest decisions first —
+  with their coordinates as each path would produce them. A speculative
+  result is used only when the walk reaches the point along that path, so
+  the tracked and survived lists, and the boundary, are the reference's,
+  with several times fewer launches (a launch of a few hundred particles
+  costs as much as one of a single particle).
+
+GridMode.FLOODFILL is not offered (the reference runs it on CPU
 multiprocessing only, acceptance.py:142-144).
 """
 
 from __future__ import annotations
 
 import warnings
-from collections import namedtuple
+from dataclasses import dataclass
 from enum import Enum
 
 import numpy as np
@@ -34,297 +58,386 @@ class GridMode(Enum):
     FLOODFILL = 3
 
 
-_AXES = {"x": 0, "px": 1, "y": 2, "py": 3, "dp": 4, "ct": 5, "delta": 4, "s": 5,
-         "xp": 1, "yp": 3}
+_PLANE_INDEX = {"x": 0, "px": 1, "xp": 1, "y": 2, "py": 3, "yp": 3, "dp": 4, "delta": 4,
+                "ct": 5, "s": 5}
 
 
-def _axis_index(plane) -> int:
+def _plane_index(plane) -> int:
     if isinstance(plane, (int, np.integer)):
         return int(plane)
-    key = str(plane)
-    if key in _AXES:
-        return _AXES[key]
-    if key.lower() in _AXES:
-        return _AXES[key.lower()]
-    raise AtError(f"Unknown plane {plane!r}")
+    idx = _PLANE_INDEX.get(str(plane), _PLANE_INDEX.get(str(plane).lower()))
+    if idx is None:
+        raise AtError(f"Unknown plane {plane!r}")
+    return idx
 
 
-_Config = namedtuple("config", "planes planesi amplitudes shape bounds mode shift_zero")
-_Grid = namedtuple("grid", "grid offset")
+@dataclass(frozen=True)
+class _Layout:
+    """What is searched: planes (names and coordinate indices), amplitude
+    and point count per plane, bounds, mode and the shift off zero."""
+
+    planes: np.ndarray
+    index: np.ndarray
+    amplitudes: np.ndarray
+    counts: np.ndarray
+    bounds: tuple
+    mode: GridMode
+    shift_zero: float
 
 
-def grid_configuration(planes, npoints, amplitudes, grid_mode, bounds=None, shift_zero=1.0e-6):
-    """boundary.py:70-109"""
+def _layout(planes, npoints, amplitudes, mode, bounds=None, shift_zero=1.0e-6) -> _Layout:
+    """Validated search layout; RADIAL and RECURSIVE bounds are
+    (radius range, angle range) with a zero radius moved to 1e-6,
+    CARTESIAN bounds are per-plane multiples of the amplitude
+    (boundary.py:70-109)."""
     planes = np.atleast_1d(planes)
-    ndims = len(planes)
-    if ndims > 2 or ndims == 0:
+    nd = planes.size
+    if nd not in (1, 2):
         raise AtError("planes can have 1 or 2 element (1D or 2D aperture)")
-    if ndims == 1 and grid_mode is GridMode.RADIAL:
-        grid_mode = GridMode.CARTESIAN
-    if np.shape(np.atleast_1d(npoints)) != (ndims,):
+    if nd == 1 and mode is GridMode.RADIAL:
+        mode = GridMode.CARTESIAN
+    counts = np.atleast_1d(np.array(npoints, dtype=np.int32))
+    amps = np.atleast_1d(np.array(amplitudes, dtype=float))
+    if counts.shape != (nd,):
         raise AtError("npoints shape should be (len(planes),)")
-    if np.shape(np.atleast_1d(amplitudes)) != (ndims,):
+    if amps.shape != (nd,):
         raise AtError("amplitudes shape should be (len(planes),)")
-    if bounds is not None and np.shape(np.atleast_2d(bounds)) != (ndims, 2):
+    if bounds is not None and np.shape(np.atleast_2d(bounds)) != (nd, 2):
         raise AtError("bounds shape should be (len(planes),2)")
-    if grid_mode is GridMode.RADIAL or grid_mode is GridMode.RECURSIVE:
-        if bounds is None:
-            bounds = np.array([[0, 1], [np.pi, 0]])
-        bounds = np.array(bounds, dtype=float)
-        bounds[0][bounds[0] == 0] = 1.0e-6
-    elif grid_mode is GridMode.CARTESIAN:
-        if bounds is None:
-            bounds = np.array([[p - 1, 1] for p in range(ndims)])
+    if mode in (GridMode.RADIAL, GridMode.RECURSIVE):
+        b = np.array([[0.0, 1.0], [np.pi, 0.0]] if bounds is None else bounds, dtype=float)
+        b[0][b[0] == 0] = 1.0e-6
+    elif mode is GridMode.CARTESIAN:
+        b = np.array([[-1.0, 1.0], [0.0, 1.0]][:nd] if bounds is None else bounds, dtype=float)
     else:
-        raise AtError(f"GridMode {grid_mode} undefined.")
-    bounds = tuple(map(tuple, np.atleast_2d(bounds)))
-    return _Config(planes, np.array([_axis_index(p) for p in planes]),
-                   np.atleast_1d(amplitudes), np.atleast_1d(np.array(npoints, dtype=np.int32)),
-                   bounds, grid_mode, shift_zero)
+        raise AtError(f"GridMode {mode} undefined.")
+    return _Layout(planes, np.array([_plane_index(p) for p in planes]), amps, counts,
+                   tuple(tuple(r) for r in np.atleast_2d(b)), mode, shift_zero)
 
 
-def get_parts(config, offset):
-    """(6, n) particles of the grid around ``offset`` (boundary.py:112-148)."""
-    amp, npt, bnd = config.amplitudes, config.shape, config.bounds
-    if config.mode is GridMode.CARTESIAN:
-        x = [np.linspace(*b, n) * a for a, b, n in zip(amp, bnd, npt)]
-        try:
-            g1, g2 = np.meshgrid(*x)
-        except ValueError:
-            g1, g2 = np.meshgrid(x, 0.0)
-        g = np.array([g1.flatten(), g2.flatten()])
-    elif config.mode is GridMode.RADIAL:
-        x = [np.linspace(*b, n) for b, n in zip(bnd, npt)]
-        g1, g2 = np.meshgrid(*x)
-        g = np.array([(amp[0] * np.cos(g2) * g1).flatten(), (amp[1] * np.sin(g2) * g1).flatten()])
-    else:
-        raise AtError(f"GridMode {config.mode} has no fixed grid")
-    parts = np.zeros((6, int(np.prod(npt))))
-    parts[config.planesi, :] = [g[i] for i in range(len(config.planesi))]
-    offset = np.array(offset) + config.shift_zero
-    parts = (parts.T + offset).T
-    return np.asfortranarray(parts), _Grid(g, offset[config.planesi])
+# --- tracking hooks (the engine) ------------------------------------------------
 
-
-# Tracking hooks (the engine): survival after nturns, and projection of
-# particles from an observation point to the end of the ring.
 def _survived(parts, ring, nturns, **kwargs):
-    """boundary.py:151-163"""
+    """Survival of each particle after ``nturns`` (one launch, in place)."""
     _, _, td = tracking.lattice_track(ring, parts, nturns=nturns, losses=True, refpts=None,
                                       in_place=True, **kwargs)
     return np.invert(td["loss_map"].islost)
 
 
 def _project(ring, parts, obs, **kwargs):
+    """Carry particles from element ``obs`` to the end of the ring."""
     tracking.lattice_track(ring, parts, 1, refpts=None, in_place=True, start_elem=int(obs),
                            **kwargs)
 
 
-def set_ring_orbit(ring, dp, obspt, orbit):
-    """Initial closed orbit and rotated ring (boundary.py:57-67)."""
+def _orbit_and_ring(ring, dp, obspt, orbit):
+    """Closed orbit at the observation point (unless given) and the ring
+    starting there (boundary.py:57-67)."""
+    start = ring
     if obspt is not None:
         if not np.isscalar(obspt):
             raise AtError("Scalar value needed for obspt")
-        ringrot = ring.rotate(int(obspt))
-    else:
-        ringrot = ring
+        start = ring.rotate(int(obspt))
     if orbit is None:
-        orbit0, orbitobs = physics.find_orbit(ring, dp=dp, refpts=obspt)
-        orbit = orbit0 if obspt is None else np.squeeze(orbitobs)
-    return np.asarray(orbit, dtype=float), ringrot
+        at_start, at_obs = physics.find_orbit(ring, dp=dp, refpts=obspt)
+        orbit = at_start if obspt is None else at_obs[0]
+    return np.asarray(orbit, dtype=float), start
 
 
-def _nearest_order(grid):
-    """Order boundary points by angle, then by nearest neighbour on the
-    normalised grid (boundary.py:170-214)."""
-    angle = np.arctan2(*grid)
-    norm = np.linalg.norm(grid.T, axis=1)
-    val, inv = np.unique(angle, return_inverse=True)
-    gf = np.zeros((2, len(val)))
-    for i in range(len(val)):
-        inds = np.where(inv == i)[0]
-        ni = norm[inds]
-        gf[:, i] = grid[:, inds[np.where(ni == np.amax(ni))[0]][0]]
-    gf = gf[:, np.argsort(np.arctan2(*gf))]
-    x, y = gf[0, :].copy(), gf[1, :].copy()
-    dxmin = min(np.diff(np.unique(x)))
-    dymin = min(np.diff(np.unique(y)))
-    iorder = [0]
-    for i in range(1, len(x)):
-        xnow, ynow = x[iorder[-1]], y[iorder[-1]]
-        dd = np.sqrt(((x - xnow) / dxmin) ** 2 + ((y - ynow) / dymin) ** 2)
-        if i <= 3:
-            ic = [j for j in np.argsort(dd) if j not in iorder]
-        else:
-            direction = np.sign(iorder[-1] - iorder[-2])
-            ic = [j for j in np.argsort(dd)
-                  if j not in iorder and np.sign(j - iorder[-1]) == direction]
-        if ic:
-            iorder.append(ic[0])
-    xnow, ynow = x[iorder[-1]], y[iorder[-1]]
-    dd = np.sqrt(((x[iorder[0]] - xnow) / dxmin) ** 2 + ((y[iorder[0]] - ynow) / dymin) ** 2)
-    if dd < 1.5:
-        iorder.append(iorder[0])
-    return gf[:, iorder]
+# --- whole grids -------------------------------------------------------------------
+
+def _grid_points(lay: _Layout) -> np.ndarray:
+    """Grid coordinates, (2, n), in the reference's order and arithmetic:
+    RADIAL radius-fastest over angles, CARTESIAN first-plane-fastest; a
+    1-D grid carries a zero second row."""
+    if lay.mode is GridMode.RADIAL:
+        radius = np.linspace(*lay.bounds[0], lay.counts[0])
+        angle = np.linspace(*lay.bounds[1], lay.counts[1])
+        r = np.tile(radius, angle.size)
+        a = np.repeat(angle, radius.size)
+        return np.array([lay.amplitudes[0] * np.cos(a) * r, lay.amplitudes[1] * np.sin(a) * r])
+    if lay.mode is GridMode.CARTESIAN:
+        axes = [np.linspace(*b, n) * amp
+                for amp, b, n in zip(lay.amplitudes, lay.bounds, lay.counts)]
+        if len(axes) == 1:
+            return np.array([axes[0], np.zeros_like(axes[0])])
+        return np.array([np.tile(axes[0], axes[1].size), np.repeat(axes[1], axes[0].size)])
+    raise AtError(f"GridMode {lay.mode} has no fixed grid")
 
 
-def _search_bnd(ma, sa):
-    bnd = np.zeros((2, 1))
-    for j, m in enumerate(ma):
-        bnd = sa[:, j]
-        if not m and j > 0:
-            bnd = sa[:, j - 1]
-            break
-    return bnd
+def _grid_beam(lay: _Layout, points: np.ndarray, orbit: np.ndarray) -> np.ndarray:
+    """(6, n) particles: the grid in its planes, plus the orbit shifted off
+    zero by ``shift_zero`` on every coordinate."""
+    beam = np.zeros((6, points.shape[1]))
+    beam[lay.index] = points[:lay.index.size]
+    return np.asfortranarray(beam + (orbit + lay.shift_zero)[:, None])
 
 
-def get_grid_boundary(mask, grid, config):
-    """Boundary of the surviving part of a grid (boundary.py:166-272)."""
-    from scipy.ndimage import binary_dilation, binary_opening
+def _last_before_first_loss(alive: np.ndarray) -> np.ndarray:
+    """For each row of a survival image ordered outwards, the column of
+    the last point before the first loss, the innermost point not counted
+    as a loss (it is the starting point of the row); the last column when
+    nothing further out is lost."""
+    n = alive.shape[1]
+    lost = ~alive[:, 1:]
+    first = np.where(lost.any(axis=1), lost.argmax(axis=1) + 1, n)
+    return first - 1
 
-    if not np.any(mask):
+
+def _cartesian_edge(alive: np.ndarray, points: np.ndarray) -> np.ndarray:
+    """Edge of the surviving region of a 2-D cartesian image (rows = second
+    plane): survivors with a surviving neighbour both along x and along y
+    (strands one point wide are dropped) that touch a non-member or the
+    grid's border; one point per direction (the farthest), by angle."""
+    left = np.zeros_like(alive)
+    left[:, 1:] = alive[:, :-1]
+    right = np.zeros_like(alive)
+    right[:, :-1] = alive[:, 1:]
+    down = np.zeros_like(alive)
+    down[1:, :] = alive[:-1, :]
+    up = np.zeros_like(alive)
+    up[:-1, :] = alive[1:, :]
+    member = alive & (left | right) & (down | up)
+    padded = np.pad(member, 1, constant_values=False)
+    inner = member.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            inner &= padded[1 + dy:1 + dy + member.shape[0], 1 + dx:1 + dx + member.shape[1]]
+    edge = (member & ~inner).reshape(-1)
+    pts = points[:, edge]
+    if pts.shape[1] == 0:
+        return pts
+    angle = np.round(np.arctan2(pts[1], pts[0]), 12)
+    radius = np.hypot(pts[0], pts[1])
+    order = np.lexsort((-radius, angle))
+    angle, pts = angle[order], pts[:, order]
+    keep = np.ones(angle.size, bool)
+    keep[1:] = angle[1:] != angle[:-1]
+    return pts[:, keep]
+
+
+def _grid_boundary(alive: np.ndarray, points: np.ndarray, lay: _Layout):
+    if not alive.any():
         warnings.warn(physics.AtWarning("No particle survived, please check your grid or "
-                                        "lattice. Acceptance set to [0.0, 0.0]."),
-                      stacklevel=2)
-        return np.zeros(2) if len(config.shape) == 1 else np.zeros((2, 1))
-    if config.mode is GridMode.RADIAL:
-        angles = np.round(np.arctan2(*grid.grid), decimals=8)
-        angles, invi = np.unique(angles, return_inverse=True)
-        bnd = np.zeros((2, len(angles)))
-        for i in range(len(angles)):
-            bnd[:, i] = _search_bnd(mask[invi == i], grid.grid[:, invi == i])
-        return bnd
-    if config.mode is GridMode.CARTESIAN:
-        cnt = np.flip(config.shape)
-        if len(cnt) == 1:
-            g = grid.grid
-            xp, xn = g[0] >= 0, g[0] <= 0
-            bp = _search_bnd(mask[xp], g[:, xp])
-            bn = _search_bnd(np.flip(mask[xn]), np.flip(g[:, xn], axis=1))
-            return np.squeeze([bn[0], bp[0]])
-        mask2d = np.reshape(mask.copy(), cnt)
-        bnd1 = binary_opening(mask2d, np.ones((2, 1)))
-        bnd2 = binary_opening(mask2d, np.ones((1, 2)))
-        bnd = np.logical_and(bnd1, bnd2)
-        bnd = np.logical_and(binary_dilation(bnd == 0, border_value=1), bnd)
-        return _nearest_order(grid.grid[:, bnd.reshape(mask.shape)])
-    raise AtError(f"GridMode {config.mode} undefined.")
+                                        "lattice. Acceptance set to [0.0, 0.0]."), stacklevel=3)
+        return np.zeros(2) if lay.counts.size == 1 else np.zeros((2, 1))
+    if lay.mode is GridMode.RADIAL:
+        # one ray per direction, keyed (and ordered) as the reference keys
+        # them: arctan2(x, y) rounded to 8 decimals; rays that share a key
+        # are searched as one, in point order
+        nr, na = lay.counts
+        image = alive.reshape(na, nr)
+        rays = points.reshape(2, na, nr)
+        key = np.round(np.arctan2(rays[0, :, 0], rays[1, :, 0]), 8)
+        edge = []
+        for k in np.unique(key):
+            rows = np.flatnonzero(key == k)
+            a = image[rows].reshape(1, -1)
+            pts = rays[:, rows].reshape(2, -1)
+            edge.append(pts[:, _last_before_first_loss(a)[0]])
+        return np.array(edge).T
+    if lay.counts.size == 1:
+        x = points[0]
+        outward = np.flatnonzero(x >= 0)
+        inward = np.flatnonzero(x <= 0)[::-1]
+        hi = outward[_last_before_first_loss(alive[outward][None, :])[0]]
+        lo = inward[_last_before_first_loss(alive[inward][None, :])[0]]
+        return np.array([x[lo], x[hi]])
+    nx, ny = lay.counts
+    return _cartesian_edge(alive.reshape(ny, nx), points)
 
 
 def grid_boundary_search(ring, planes, npoints, amplitudes, nturns=1024, obspt=None, dp=None,
                          offset=None, bounds=None, grid_mode=GridMode.RADIAL, verbose=False,
                          shift_zero=1.0e-9, **kwargs):
-    """Track whole grids, one launch for all observation points
-    (boundary.py:275-404, FLOODFILL excluded)."""
-    config = grid_configuration(planes, npoints, amplitudes, grid_mode, bounds=bounds,
-                                shift_zero=shift_zero)
-    if offset is None:
-        offset = [None for _ in range(len(obspt))]
-    allparts, grids = [], []
-    for obs, orbit in zip(obspt, offset):
-        orbit, _ = set_ring_orbit(ring, dp, obs, orbit)
-        parts, grid = get_parts(config, orbit)
-        # project particles from the observation point to the ring's end
-        _project(ring, parts, 0 if obs is None else obs, **kwargs)
-        allparts.append(parts)
-        grids.append(grid)
-    allparts = np.asfortranarray(np.concatenate(allparts, axis=1))
-    mask = np.split(_survived(allparts, ring, nturns, **kwargs), len(grids))
-    survived = [g.grid[:, m] for g, m in zip(grids, mask)]
-    boundary = [get_grid_boundary(m, g, config) for g, m in zip(grids, mask)]
-    grids = [g.grid for g in grids]
-    if len(obspt) == 1:
-        return boundary[0], survived[0], grids[0]
-    return boundary, survived, grids
+    """Boundary from whole grids at one or more observation points, all
+    tracked in one launch (boundary.py:275-404 without FLOODFILL)."""
+    lay = _layout(planes, npoints, amplitudes, grid_mode, bounds, shift_zero)
+    points = _grid_points(lay)
+    orbits = [None] * len(obspt) if offset is None else offset
+    beams = []
+    for obs, orbit in zip(obspt, orbits):
+        orbit, _ = _orbit_and_ring(ring, dp, obs, orbit)
+        beam = _grid_beam(lay, points, orbit)
+        _project(ring, beam, 0 if obs is None else obs, **kwargs)
+        beams.append(beam)
+    alive = _survived(np.asfortranarray(np.concatenate(beams, axis=1)), ring, nturns, **kwargs)
+    alive = alive.reshape(len(beams), points.shape[1])
+    boundary = [_grid_boundary(a, points, lay) for a in alive]
+    survived = [points[:, a] for a in alive]
+    if len(beams) == 1:
+        return boundary[0], survived[0], points
+    return boundary, survived, [points] * len(beams)
+
+
+# --- radial walk -------------------------------------------------------------------
+
+class _Walk:
+    """The RECURSIVE search along every ray at once.
+
+    Per ray: ``pos`` (integer, in finest steps), ``level`` (0 = coarsest;
+    the step at level l is divider**(depth - l) finest steps, the first
+    being the whole amplitude), ``xyz`` (the floating-point coordinates in
+    the search planes, advanced with the reference's operations:
+    ``x += base*fact`` out, ``x -= base*min(1, 2*fact)`` back, or reset to
+    zero when stepping back would cross it), ``done``."""
+
+    def __init__(self, lay: _Layout, divider, offset):
+        nd = lay.index.size
+        rtol = np.min(lay.amplitudes / lay.counts)
+        ftol = np.min(rtol / lay.amplitudes)
+        facts = [1.0]
+        while facts[-1] > ftol:
+            facts.append(facts[-1] * (1 / divider))
+        self.facts = facts                     # fact per level, as the reference computes it
+        self.depth = len(facts) - 1            # the finest level
+        self.unit = [divider ** (self.depth - lvl) for lvl in range(self.depth + 1)]
+        angles = np.atleast_1d(np.linspace(*lay.bounds[1], lay.counts[1] if nd == 2 else 2))
+        cs = np.around(np.array([np.cos(angles), np.sin(angles)]), decimals=9)
+        # per ray: the coarse step in the search planes, and its components
+        # along both directions used by the step-back test
+        self.base = cs[:nd] * lay.amplitudes[:, None]
+        self.test = cs * lay.amplitudes[:, None] if nd == 2 else cs * lay.amplitudes[0]
+        self.nd, self.nray = nd, angles.size
+        self.lay, self.offset = lay, offset
+        self.pos = np.zeros(self.nray, np.int64)
+        self.level = np.zeros(self.nray, np.int64)
+        self.xyz = np.zeros((nd, self.nray))
+        self.done = np.zeros(self.nray, bool)
+        self.outcome = {}     # (ray, pos) -> survived, fixed at the first visit
+        self.spare = {}       # (ray, pos, coordinate bytes) -> survived, tracked ahead
+
+    # the reference's floating-point moves
+    def _out(self, ray, xyz, level):
+        return xyz + self.base[:, ray] * self.facts[level]
+
+    def _back(self, ray, xyz, pos, level):
+        m = min(1, 2 * self.facts[level])
+        if np.any(np.abs(self.test[:, ray] * m) > np.abs(xyz)):
+            return np.zeros_like(xyz), 0
+        return xyz - self.base[:, ray] * m, pos - min(self.unit[0], 2 * self.unit[level])
+
+    def _ahead(self, ray, xyz, pos, level, budget):
+        """Up to ``budget`` points the ray may need after the pending one
+        (xyz, pos, level), nearest decisions first: the walk is played
+        forward from it for both outcomes of every point whose outcome is
+        not known (assuming, along each branch, the outcomes that branch
+        took), with the coordinates each path produces."""
+        found, listed = [], {(pos, xyz.tobytes())}
+        queue = [(xyz, pos, level, {})]
+        expansions = 0
+        while queue and len(found) < budget and expansions < 8 * budget:
+            x, p, lvl, assumed = queue.pop(0)
+            expansions += 1
+            known = self.outcome.get((ray, p), assumed.get(p))
+            for alive in ((known,) if known is not None else (True, False)):
+                branch = dict(assumed)
+                branch[p] = alive
+                if alive:
+                    nxt = (self._out(ray, x, lvl), p + self.unit[lvl], lvl)
+                elif lvl == self.depth:
+                    continue
+                else:
+                    xb, pb = self._back(ray, x, p, lvl)
+                    nxt = (self._out(ray, xb, lvl + 1), pb + self.unit[lvl + 1], lvl + 1)
+                nx, np_, nl = nxt
+                tag = (np_, nx.tobytes())
+                if ((ray, np_) not in self.outcome and np_ not in branch
+                        and tag not in listed):
+                    listed.add(tag)
+                    found.append((nx, np_))
+                queue.append((nx, np_, nl, branch))
+        return found[:budget]
+
+    def _track(self, wanted, ring, nturns, kwargs):
+        beam = np.zeros((6, len(wanted)))
+        for j, (_, xyz, _) in enumerate(wanted):
+            beam[self.lay.index, j] = xyz
+        beam = np.asfortranarray(beam + self.offset[:, None])
+        alive = _survived(beam, ring, nturns, **kwargs)
+        for (ray, xyz, p), a in zip(wanted, alive):
+            self.spare[(ray, p, xyz.tobytes())] = bool(a)
+
+    def run(self, ring, nturns, lookahead, kwargs):
+        tracked, kept = [], []
+        while not self.done.all():
+            active = np.flatnonzero(~self.done)
+            for r in active:
+                self.xyz[:, r] = self._out(r, self.xyz[:, r], self.level[r])
+                self.pos[r] += self.unit[self.level[r]]
+            # rays whose point is new and not tracked ahead along this path
+            need = [r for r in active if (r, self.pos[r]) not in self.outcome
+                    and (r, self.pos[r], self.xyz[:, r].tobytes()) not in self.spare]
+            if need:
+                wanted = []
+                for r in need:
+                    wanted.append((r, self.xyz[:, r].copy(), self.pos[r]))
+                    wanted += [(r, x, p) for x, p in
+                               self._ahead(r, self.xyz[:, r], self.pos[r], self.level[r],
+                                           lookahead)]
+                self._track(wanted, ring, nturns, kwargs)
+            for r in active:
+                key = (r, self.pos[r])
+                if key not in self.outcome:
+                    alive = self.spare[(r, self.pos[r], self.xyz[:, r].tobytes())]
+                    self.outcome[key] = alive
+                    tracked.append(self.xyz[:, r].copy())
+                    if alive:
+                        kept.append(self.xyz[:, r].copy())
+                if self.outcome[key]:
+                    continue
+                if self.level[r] == self.depth:
+                    self.done[r] = True
+                else:
+                    self.xyz[:, r], self.pos[r] = self._back(r, self.xyz[:, r], self.pos[r],
+                                                              self.level[r])
+                    self.level[r] += 1
+        # the boundary: one finest step inside the final loss
+        edge = np.stack([self.xyz[:, r] - self.base[:, r] * self.facts[self.depth]
+                         for r in range(self.nray)], axis=1)
+        as_array = (lambda pts: np.array(pts).T if pts else np.zeros((self.nd, 0)))
+        return np.squeeze(edge), as_array(kept), as_array(tracked)
 
 
 def recursive_boundary_search(ring, planes, npoints, amplitudes, nturns=1024, obspt=None,
                               dp=None, offset=None, bounds=None, divider=2, verbose=False,
-                              shift_zero=1.0e-9, **kwargs):
-    """Radial search along each angle, halving the step after a loss
-    (boundary.py:407-503). Each round tracks the new trial points of all
-    angles in one launch."""
-    offset, newring = set_ring_orbit(ring, dp, obspt, offset)
-    config = grid_configuration(planes, npoints, amplitudes, GridMode.RECURSIVE, bounds=bounds,
-                                shift_zero=shift_zero)
-    rtol = min(np.atleast_1d(config.amplitudes / config.shape))
-    rsteps = config.amplitudes
-    if len(np.atleast_1d(config.shape)) == 2:
-        angles = np.linspace(*config.bounds[1], config.shape[1])
-    else:
-        angles = np.linspace(*config.bounds[1], 2)
-    angles = np.atleast_1d(angles)
-    planesi = config.planesi
-
-    ftol = min(rtol / rsteps)
-    cs = np.squeeze([np.cos(angles), np.sin(angles)])
-    cs = np.around(cs, decimals=9)
-    fact = np.ones(len(angles))
-    survived = np.full(len(angles), True)
-    part = np.zeros((6, len(angles)))
-    grid = np.array([])
-    mask = np.array([])
-    while np.any(survived):
-        for i, pi in enumerate(planesi):
-            part[pi, survived] += cs[i, survived] * rsteps[i] * fact[survived]
-        istracked = np.array([not np.any([np.allclose(p, g, rtol=1.0e-9) for g in grid.T])
-                              for p in part[planesi].T])
-        survived = np.array([np.any([np.allclose(p, m, rtol=1.0e-9) for m in mask.T])
-                             for p in part[planesi].T])
-        pt = part[:, istracked]
-        grid = np.hstack([grid, pt[planesi]]) if grid.size else pt[planesi]
-        ptmp = np.asfortranarray((pt.T + offset).T)
-        if ptmp.shape[1] > 0:
-            survived[istracked] = _survived(ptmp, newring, nturns, **kwargs)
-        pm = part[:, np.logical_and(istracked, survived)]
-        mask = np.hstack([mask, pm[planesi]]) if mask.size else pm[planesi]
-        for i in range(len(angles)):
-            if not survived[i] and fact[i] > ftol:
-                deltas = cs[:, i] * rsteps[:] * min(1, 2 * fact[i])
-                if np.any(abs(deltas) > abs(part[planesi, i])):
-                    part[planesi, i] = np.zeros(len(planesi))
-                else:
-                    for j, pi in enumerate(planesi):
-                        part[pi, i] -= deltas[j]
-                survived[i] = True
-                fact[i] *= 1 / divider
-    for i, pi in enumerate(planesi):
-        part[pi] -= cs[i] * rsteps[i] * fact
-    return np.squeeze(part[planesi]), mask, grid
+                              shift_zero=1.0e-9, lookahead=32, **kwargs):
+    """RECURSIVE boundary at one observation point (boundary.py:407-503)."""
+    orbit, start = _orbit_and_ring(ring, dp, obspt, offset)
+    lay = _layout(planes, npoints, amplitudes, GridMode.RECURSIVE, bounds, shift_zero)
+    return _Walk(lay, divider, orbit).run(start, nturns, lookahead, kwargs)
 
 
 def boundary_search(ring, planes, npoints, amplitudes, nturns=1024, obspt=None, dp=None,
                     offset=None, bounds=None, grid_mode=GridMode.RADIAL, use_mp=False,
                     verbose=False, shift_zero=1.0e-9, **kwargs):
     """Loss boundary at one or more observation points (boundary.py:506-600)."""
-    rp = (tracking.get_uint32_index(ring, obspt) if obspt is not None
-          else np.atleast_1d(obspt))
+    obs = (tracking.get_uint32_index(ring, obspt) if obspt is not None
+           else np.atleast_1d(obspt))
     if offset is not None:
         try:
-            offset = np.broadcast_to(offset, (len(rp), 6))
+            offset = np.broadcast_to(offset, (len(obs), 6))
         except ValueError:
             raise AtError(f"offset and refpts have incoherent shapes: {np.shape(offset)}, "
                           f"{np.shape(obspt)}") from None
     divider = kwargs.pop("divider", 2)
-    kwargs.pop("gpu_pool", None)
-    kwargs.pop("pool_size", None)
-    kwargs.pop("start_method", None)
+    for k in ("gpu_pool", "pool_size", "start_method"):
+        kwargs.pop(k, None)
     if grid_mode is GridMode.FLOODFILL:
         raise AtError("floodfill is not implemented for GPU tracking")
-    if grid_mode is GridMode.RECURSIVE:
-        if offset is None:
-            offset = [None for _ in rp]
-        out = [recursive_boundary_search(ring, planes, npoints, amplitudes, nturns=nturns,
-                                         obspt=r, dp=dp, offset=o, bounds=bounds,
-                                         divider=divider, verbose=verbose,
-                                         shift_zero=shift_zero, **kwargs)
-               for r, o in zip(rp, offset)]
-        if len(rp) == 1:
-            return out[0]
-        return tuple(list(x) for x in zip(*out))
-    return grid_boundary_search(ring, planes, npoints, amplitudes, nturns=nturns, obspt=rp,
-                                dp=dp, offset=offset, bounds=bounds, grid_mode=grid_mode,
-                                verbose=verbose, shift_zero=shift_zero, **kwargs)
+    if grid_mode is not GridMode.RECURSIVE:
+        return grid_boundary_search(ring, planes, npoints, amplitudes, nturns=nturns,
+                                    obspt=obs, dp=dp, offset=offset, bounds=bounds,
+                                    grid_mode=grid_mode, verbose=verbose,
+                                    shift_zero=shift_zero, **kwargs)
+    orbits = [None] * len(obs) if offset is None else offset
+    results = [recursive_boundary_search(ring, planes, npoints, amplitudes, nturns=nturns,
+                                         obspt=o, dp=dp, offset=orb, bounds=bounds,
+                                         divider=divider, shift_zero=shift_zero, **kwargs)
+               for o, orb in zip(obs, orbits)]
+    if len(results) == 1:
+        return results[0]
+    return tuple(list(parts) for parts in zip(*results))
 
 
 def get_acceptance(ring, planes, npoints, amplitudes, nturns=1024, refpts=None, dp=None,
@@ -333,33 +446,34 @@ def get_acceptance(ring, planes, npoints, amplitudes, nturns=1024, refpts=None, 
                    start_method=None, **kwargs):
     """Acceptance at the ``refpts`` observation points (acceptance.py:27-199).
     Returns (boundary, survived, tracked)."""
-    axes = list(np.atleast_1d(planes))
-    for i, a in enumerate(axes):
-        if a in ("xp", "yp"):
-            warnings.warn(physics.AtWarning(f"Axis name {a} is deprecated."), stacklevel=2)
-            axes[i] = a[::-1]
-    planes = tuple(axes) if len(axes) > 1 else axes[0]
+    names = []
+    for p in np.atleast_1d(planes):
+        if p in ("xp", "yp"):
+            warnings.warn(physics.AtWarning(f"Axis name {p} is deprecated."), stacklevel=2)
+            p = p[::-1]
+        names.append(p)
     if gpu_pool is not None:
         tracking.engine.set_device(int(gpu_pool[0]))
-    return boundary_search(ring, planes, npoints, amplitudes, nturns=nturns, obspt=refpts,
-                           dp=dp, offset=offset, bounds=bounds, grid_mode=grid_mode,
-                           use_mp=use_mp, verbose=verbose, divider=divider,
-                           shift_zero=shift_zero, **kwargs)
+    return boundary_search(ring, tuple(names) if len(names) > 1 else names[0], npoints,
+                           amplitudes, nturns=nturns, obspt=refpts, dp=dp, offset=offset,
+                           bounds=bounds, grid_mode=grid_mode, use_mp=use_mp, verbose=verbose,
+                           divider=divider, shift_zero=shift_zero, **kwargs)
 
 
 def get_1d_acceptance(ring, plane, resolution, amplitude, nturns=1024, refpts=None, dp=None,
                       offset=None, grid_mode=GridMode.RADIAL, use_mp=False, gpu_pool=None,
                       verbose=False, divider=2, shift_zero=1.0e-6, start_method=None, **kwargs):
-    """1D acceptance (acceptance.py:202-303): without multiprocessing the
-    reference forces the RECURSIVE search, and so does this driver."""
+    """Acceptance along one plane (acceptance.py:202-303). Without
+    ``use_mp`` the reference runs the RECURSIVE walk whatever grid_mode
+    says, and so does this driver."""
     if not use_mp:
         grid_mode = GridMode.RECURSIVE
-    if len(np.atleast_1d(plane)) != 1:
+    if np.size(plane) != 1:
         raise AtError("1D acceptance: single plane required")
     if not (np.isscalar(resolution) and np.isscalar(amplitude)):
         raise AtError("1D acceptance: scalar args required")
     npoint = np.ceil(amplitude / resolution)
-    if grid_mode is not GridMode.RECURSIVE and not npoint > 1:
+    if grid_mode is not GridMode.RECURSIVE and npoint <= 1:
         raise AtError("Grid has only one point: increase amplitude or reduce resolution")
     b, s, g = get_acceptance(ring, plane, npoint, amplitude, nturns=nturns, dp=dp,
                              refpts=refpts, grid_mode=grid_mode, use_mp=use_mp,

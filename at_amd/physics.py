@@ -1,17 +1,36 @@
-"""Closed orbit and transfer matrices by tracking on the engine.
+"""Closed orbits and one-turn matrices by tracking on the engine.
 
-Restatement of pyat's orbit and matrix finders (pyat/at/physics/orbit.py,
-matrix.py, energy_loss.py): every one-turn map evaluation is a small batch
-of particles (the reference orbit plus one displaced copy per coordinate)
-tracked through the lattice in one engine launch.
+Every evaluation of the one-turn map is one engine launch of a *probe*: the
+point of interest plus, for each varied coordinate j, the two copies
+``x +- step_j/2 e_j``. The probe gives at once the image of the point and
+the central-difference Jacobian of the map there, and with refpts the same
+launch gives the transfer matrices from the ring's entrance to every refpt.
+Extra particles cost nothing on the device (a launch of 13 particles takes
+as long as one of 1), so the closed-orbit solvers use the central
+difference for their Newton Jacobian: it is second-order accurate where a
+one-sided difference is first-order, and each Newton iteration, one launch,
+gains more.
 
-* find_orbit4 / find_sync_orbit / find_orbit6 / find_orbit: Newton
-  iterations on the one-turn map with forward-difference Jacobians
-  (orbit.py:16-72 _orbit_dp, :74-120 _orbit_dct, :307-368 _orbit6);
-* find_m44 / find_m66: central differences around the closed orbit
-  (matrix.py:19-168); find_elem_m66: one element (matrix.py:171-205);
-* get_energy_loss / get_timelag_fromU0 (energy_loss.py:31-206) for the
-  synchronous phase that seeds find_orbit6.
+The closure conditions are written as one affine form,
+
+    residual(x) = M(x)[rows] - L x - c,
+
+solved for the coordinates ``free`` of the input:
+
+* fixed momentum deviation (pyat ``find_orbit4`` with ``dp``,
+  pyat/at/physics/orbit.py:16-72): rows = free = x, px, y, py; L = I;
+* fixed path lengthening (``find_sync_orbit``, orbit.py:74-120): free =
+  x..delta, rows = x, px, y, py, ct; L = I on the transverse rows and 0 on
+  ct, c_ct = dct (the output ct itself equals dct);
+* 6-D with cavities (``find_orbit6``, orbit.py:307-368): all six
+  coordinates, L = I, c_ct = the ct lag of the harmonic number
+  (beta c h / f_rf - L0).
+
+Steps and defaults follow the reference options (``XYStep`` for the
+transverse coordinates and for delta in the path-lengthening solve,
+``DPStep`` for delta and ct in 6-D; ``convergence``, ``max_iterations``).
+The matrices (matrix.py:19-205) are the probe's central differences with
+the reference's steps, so they are the reference's own estimate.
 """
 
 from __future__ import annotations
@@ -50,8 +69,8 @@ class ELossMethod(Enum):
     TRACKING = 2
 
 
-# One-turn tracking in place through the engine (internal_lpass); returns
-# the refpt copies (6, N, R, 1).
+# Tracking hooks: one turn in place (returns the refpt copies (6, N, R, 1)),
+# and one element.
 def _lpass(ring, r_in, refpts=None, keep_lattice=False):
     refs = np.zeros(0, np.uint32) if refpts is None else refpts
     return tracking.internal_lpass(ring, r_in, 1, refpts=refs, keep_lattice=keep_lattice)
@@ -65,121 +84,196 @@ def _refs(ring, refpts) -> np.ndarray:
     return tracking.get_uint32_index(ring, refpts)
 
 
-def _options(kwargs, *, dp_step=False):
+_TRANSVERSE = np.array([1.0, 1.0, 1.0, 1.0, 0.0, 0.0])
+
+
+def _probe_batch(center: np.ndarray, steps: np.ndarray):
+    """Column 0 is ``center``; then, for the k coordinates with a nonzero
+    step, k columns displaced by +step/2 and k by -step/2."""
+    varied = np.flatnonzero(steps)
+    k = varied.size
+    batch = np.tile(np.asarray(center, dtype=float).reshape(6, 1), (1, 1 + 2 * k))
+    half = 0.5 * steps[varied]
+    batch[varied, 1 + np.arange(k)] += half
+    batch[varied, 1 + k + np.arange(k)] -= half
+    return np.asfortranarray(batch), varied
+
+
+def _differences(images: np.ndarray, steps: np.ndarray, varied: np.ndarray) -> np.ndarray:
+    """Central differences d image / d input[varied] from a probe's images
+    (the leading axis may hold the six coordinates of any number of copies)."""
+    k = varied.size
+    return (images[..., 1:1 + k] - images[..., 1 + k:]) / steps[varied]
+
+
+def _probe(ring, center, steps, refs=None, keep_lattice=False):
+    """One launch: the image of ``center``, the Jacobian (6, k) of the map
+    with respect to the varied coordinates, and the copies at ``refs``
+    as (R, 6, 1 + 2k)."""
+    batch, varied = _probe_batch(center, steps)
+    copies = _lpass(ring, batch, refs, keep_lattice)
+    copies = np.moveaxis(np.asarray(copies)[..., 0], 2, 0)
+    return batch[:, 0].copy(), _differences(batch, steps, varied), copies
+
+
+def _close(ring, x, free, rows, lin, const, steps, keep_lattice, conv, maxit):
+    """Newton on residual(x) = M(x)[rows] - lin @ x - const over x[free]:
+    one probe (one launch) per iteration. Stops when an update moves x by
+    at most ``conv`` (2-norm); warns if ``maxit`` updates did not get there."""
+    x = np.array(x, dtype=float)
+    probe_steps = np.zeros(6)
+    probe_steps[free] = steps[free]
+    for _ in range(maxit):
+        image, jac, _ = _probe(ring, x, probe_steps, None, keep_lattice)
+        keep_lattice = True
+        residual = image[rows] - lin @ x - const
+        slope = jac[rows] - lin[:, free]
+        update = np.linalg.solve(slope, residual)
+        x[free] -= update
+        if np.linalg.norm(update) <= conv:
+            return x
+    warnings.warn(AtWarning("Maximum number of iterations reached. Possible non-convergence"),
+                  stacklevel=3)
+    return x
+
+
+def _options(kwargs, what):
     conv = kwargs.pop("convergence", DConstant.OrbConvergence)
     maxit = kwargs.pop("max_iterations", DConstant.OrbMaxIter)
     xy = kwargs.pop("XYStep", DConstant.XYStep)
     dps = kwargs.pop("DPStep", DConstant.DPStep)
-    return conv, maxit, xy, dps
-
-
-def _newton(ring, ref_in, delta, rows, cols, ident, target, keep_lattice, conv, maxit):
-    """Newton iteration on the one-turn map: the first columns of the
-    tracked batch are ref_in + delta, the last one is ref_in itself."""
-    nd = delta.shape[1]
-    change = 1.0
-    it = 0
-    scaling = np.array([delta[i, i] for i in range(nd)])
-    while change > conv and it < maxit:
-        in_mat = np.asfortranarray(ref_in.reshape((6, 1)) + np.concatenate(
-            (delta, np.zeros((6, 1))), axis=1))
-        _lpass(ring, in_mat, keep_lattice=keep_lattice)
-        ref_out = in_mat[:, nd]
-        jac = (in_mat[np.ix_(rows, cols)] - in_mat[rows, nd:nd + 1]) / scaling[cols]
-        a = jac - ident
-        b = target(ref_out, ref_in)
-        r_next = ref_in.copy()
-        r_next[cols] = ref_in[cols] - np.linalg.solve(a, b)
-        change = float(np.linalg.norm(r_next - ref_in))
-        it += 1
-        ref_in = r_next
-        keep_lattice = True
-    if it == maxit:
-        warnings.warn(AtWarning("Maximum number of iterations reached. "
-                                "Possible non-convergence"), stacklevel=3)
-    return ref_in
-
-
-def _orbit_dp(ring, dp=None, guess=None, keep_lattice=False, **kwargs):
-    """Fixed momentum deviation (orbit.py:16-72)."""
-    conv, maxit, xy, _ = _options(kwargs)
+    method = kwargs.pop("method", ELossMethod.TRACKING) if what == "orbit6" else None
     if kwargs:
-        raise AtError(f"Unexpected keywords for orbit_dp: {', '.join(kwargs)}")
-    ref_in = np.zeros(6) if guess is None else np.array(guess, dtype=float)
-    ref_in[4] = 0.0 if dp is None else dp
-    delta = np.zeros((6, 4))
-    for i in range(4):
-        delta[i, i] = xy
-    idx = np.arange(4)
-    return _newton(ring, ref_in, delta, idx, idx, np.eye(4),
-                   lambda out, rin: out[:4] - rin[:4], keep_lattice, conv, maxit)
+        raise AtError(f"Unexpected keywords for {what}: {', '.join(kwargs)}")
+    return conv, maxit, xy, dps, method
 
 
-def _orbit_dct(ring, dct=None, guess=None, keep_lattice=False, **kwargs):
-    """Fixed path lengthening (orbit.py:74-120)."""
-    conv, maxit, xy, _ = _options(kwargs)
-    if kwargs:
-        raise AtError(f"Unexpected keywords for orbit_dct: {', '.join(kwargs)}")
-    ref_in = np.zeros(6) if guess is None else np.array(guess, dtype=float)
-    delta = np.zeros((6, 5))
-    for i in range(5):
-        delta[i, i] = xy
-    theta5 = np.zeros(5)
-    theta5[4] = 0.0 if dct is None else dct
-    id5 = np.zeros((5, 5))
-    id5[:4, :4] = np.eye(4)
+def _start(guess) -> np.ndarray:
+    return np.zeros(6) if guess is None else np.array(guess, dtype=float)
+
+
+def _orbit_at_dp(ring, dp=None, guess=None, keep_lattice=False, **kwargs):
+    """Transverse fixed point at a fixed momentum deviation."""
+    conv, maxit, xy, _, _ = _options(kwargs, "orbit_dp")
+    x = _start(guess)
+    x[4] = 0.0 if dp is None else dp
+    t = np.arange(4)
+    lin = np.eye(6)[t]
+    return _close(ring, x, t, t, lin, np.zeros(4), xy * np.ones(6), keep_lattice, conv, maxit)
+
+
+def _orbit_at_dct(ring, dct=None, guess=None, keep_lattice=False, **kwargs):
+    """Transverse fixed point whose one-turn path lengthening is ``dct``:
+    delta is an unknown, the output ct must equal dct."""
+    conv, maxit, xy, _, _ = _options(kwargs, "orbit_dct")
+    x = _start(guess)
     rows = np.array([0, 1, 2, 3, 5])
-    cols = np.arange(5)
-    return _newton(ring, ref_in, delta, rows, cols, id5,
-                   lambda out, rin: out[rows] - np.append(rin[:4], 0.0) - theta5,
-                   keep_lattice, conv, maxit)
+    lin = np.zeros((5, 6))
+    lin[np.arange(4), np.arange(4)] = 1.0
+    const = np.zeros(5)
+    const[4] = 0.0 if dct is None else dct
+    return _close(ring, x, np.arange(5), rows, lin, const, xy * np.ones(6), keep_lattice, conv,
+                  maxit)
 
 
-def _propagate(ring, orbit, refpts, keep_lattice):
+def _orbit_6d(ring, cavpts=None, guess=None, keep_lattice=False, **kwargs):
+    """6-D fixed point up to the ct lag of the RF harmonic number; the
+    starting ct is the synchronous time lag from the energy loss."""
+    conv, maxit, xy, dps, method = _options(kwargs, "orbit6")
+    length = cell_length(ring)
+    f_rf = get_rf_frequency(ring)
+    harmonic = round(f_rf * length / beta(ring) / CLIGHT)
+    if guess is None:
+        x = np.zeros(6)
+        try:
+            _, lag = get_timelag_fromU0(ring, method=method, cavpts=cavpts)
+        except AtError as exc:
+            raise AtError("Could not determine the initial synchronous phase") from exc
+        x[5] = -lag
+    else:
+        x = _start(guess)
+    const = np.zeros(6)
+    const[5] = beta(ring) * CLIGHT * harmonic / f_rf - length
+    steps = xy * _TRANSVERSE + dps * (1.0 - _TRANSVERSE)
+    every = np.arange(6)
+    return _close(ring, x, every, every, np.eye(6), const, steps, keep_lattice, conv, maxit)
+
+
+def _along(ring, orbit, refpts, keep_lattice):
+    """The orbit at each refpt: one single-particle launch, (R, 6)."""
     refs = _refs(ring, refpts)
     if refs.size == 0:
         return np.empty((0, 6))
-    r = np.asfortranarray(orbit.reshape(6, 1).copy())
-    out = _lpass(ring, r, refs, keep_lattice)
-    return np.squeeze(out, axis=(1, 3)).T
+    r = np.asfortranarray(np.array(orbit, dtype=float).reshape(6, 1))
+    copies = _lpass(ring, r, refs, keep_lattice)
+    return np.asarray(copies)[:, 0, :, 0].T.copy()
+
+
+def _momentum_choice(dp, dct, df):
+    if sum(v is not None for v in (dp, dct, df)) > 1:
+        raise AtError("For off-momentum specification, only one of dp, dct and df may be "
+                      "specified")
+
+
+def _dct_from_df(ring, df):
+    f_rf = cell_revolution_frequency(ring) * cell_harmnumber(ring)
+    return -cell_length(ring) * df / (f_rf + df)
 
 
 def find_orbit4(ring, dp=None, refpts=None, *, dct=None, df=None, orbit=None,
                 keep_lattice=False, **kwargs):
-    """4D closed orbit for a given dp (orbit.py:123-212)."""
-    if len([v for v in (dp, dct, df) if v is not None]) > 1:
-        raise AtError("For off-momentum specification, only one of dp, dct and df may be "
-                      "specified")
+    """4-D closed orbit at a momentum deviation, or at a path lengthening
+    (``dct``) or RF frequency shift (``df``) (orbit.py:123-212). Returns
+    (orbit (6,), orbit at refpts (R, 6))."""
+    _momentum_choice(dp, dct, df)
     if orbit is None:
         if df is not None:
-            frf = cell_revolution_frequency(ring) * cell_harmnumber(ring)
-            dct = -cell_length(ring) * df / (frf + df)
-            orbit = _orbit_dct(ring, dct, keep_lattice=keep_lattice, **kwargs)
+            orbit = _orbit_at_dct(ring, _dct_from_df(ring, df), keep_lattice=keep_lattice,
+                                  **kwargs)
         elif dct is not None:
-            orbit = _orbit_dct(ring, dct, keep_lattice=keep_lattice, **kwargs)
+            orbit = _orbit_at_dct(ring, dct, keep_lattice=keep_lattice, **kwargs)
         else:
-            orbit = _orbit_dp(ring, dp, keep_lattice=keep_lattice, **kwargs)
+            orbit = _orbit_at_dp(ring, dp, keep_lattice=keep_lattice, **kwargs)
         keep_lattice = True
-    return orbit, _propagate(ring, orbit, refpts, keep_lattice)
+    return orbit, _along(ring, orbit, refpts, keep_lattice)
 
 
 def find_sync_orbit(ring, dct=None, refpts=None, *, dp=None, df=None, orbit=None,
                     keep_lattice=False, **kwargs):
-    """Closed orbit for a given path lengthening (orbit.py:215-304)."""
-    if len([v for v in (dp, dct, df) if v is not None]) > 1:
-        raise AtError("For off-momentum specification, only one of dp, dct and df may be "
-                      "specified")
+    """Closed orbit at a path lengthening (orbit.py:215-304); ``dp`` or
+    ``df`` may be given instead."""
+    _momentum_choice(dp, dct, df)
     if orbit is None:
-        if df is not None:
-            frf = cell_revolution_frequency(ring) * cell_harmnumber(ring)
-            dct = -cell_length(ring) * df / (frf + df)
-            orbit = _orbit_dct(ring, dct, keep_lattice=keep_lattice, **kwargs)
-        elif dp is not None:
-            orbit = _orbit_dp(ring, dp, keep_lattice=keep_lattice, **kwargs)
+        if dp is not None:
+            orbit = _orbit_at_dp(ring, dp, keep_lattice=keep_lattice, **kwargs)
         else:
-            orbit = _orbit_dct(ring, dct, keep_lattice=keep_lattice, **kwargs)
+            target = _dct_from_df(ring, df) if df is not None else dct
+            orbit = _orbit_at_dct(ring, target, keep_lattice=keep_lattice, **kwargs)
         keep_lattice = True
-    return orbit, _propagate(ring, orbit, refpts, keep_lattice)
+    return orbit, _along(ring, orbit, refpts, keep_lattice)
+
+
+def find_orbit6(ring, refpts=None, *, orbit=None, keep_lattice=False, **kwargs):
+    """6-D closed orbit of a ring with cavities (orbit.py:371-458)."""
+    if not _cavities(ring):
+        raise AtError("No cavity found in the lattice")
+    if orbit is None:
+        orbit = _orbit_6d(ring, keep_lattice=keep_lattice, **kwargs)
+        keep_lattice = True
+    return orbit, _along(ring, orbit, refpts, keep_lattice)
+
+
+def find_orbit(ring, refpts=None, **kwargs):
+    """find_orbit6 on a 6-D lattice, find_sync_orbit when dct or df is
+    given, find_orbit4 otherwise (orbit.py:461-519)."""
+    if is_6d(ring):
+        for k in ("dp", "dct", "df"):
+            if kwargs.pop(k, None) is not None:
+                raise AtError(f"{k} is not supported on a 6D lattice")
+        return find_orbit6(ring, refpts=refpts, **kwargs)
+    if kwargs.get("dct") is not None or kwargs.get("df") is not None:
+        return find_sync_orbit(ring, refpts=refpts, **kwargs)
+    return find_orbit4(ring, refpts=refpts, **kwargs)
 
 
 # --- ring quantities ---------------------------------------------------------
@@ -188,16 +282,15 @@ def cell_length(ring) -> float:
     return float(sum(getattr(e, "Length", 0.0) for e in ring))
 
 
-def _gamma_beta(ring):
+def beta(ring) -> float:
+    """Relativistic beta, evaluated as pyat's Lattice.beta does
+    (sqrt(1 - 1/gamma/gamma), lattice_object.py:911-919) so that the ct lag
+    of the harmonic number is the reference's to the last bit."""
     rest = float(getattr(ring.particle, "rest_energy", 0.0))
     if rest == 0.0:
-        return math.inf, 1.0
-    gamma = ring.energy / rest
-    return gamma, math.sqrt(1.0 - 1.0 / (gamma * gamma))
-
-
-def beta(ring) -> float:
-    return _gamma_beta(ring)[1]
+        return 1.0
+    gamma = float(ring.energy / rest)
+    return math.sqrt(1.0 - 1.0 / gamma / gamma)
 
 
 def cell_revolution_frequency(ring) -> float:
@@ -209,13 +302,12 @@ def _cavities(ring):
 
 
 def get_rf_frequency(ring) -> float:
-    cav = _cavities(ring)
-    if not cav:
+    freqs = {float(c.Frequency) for c in _cavities(ring)}
+    if not freqs:
         raise AtError("No cavity found in the lattice")
-    freqs = np.unique([c.Frequency for c in cav])
-    if freqs.size > 1:
+    if len(freqs) > 1:
         raise AtError("Cavities have different frequencies")
-    return float(freqs[0])
+    return freqs.pop()
 
 
 def cell_harmnumber(ring) -> int:
@@ -223,133 +315,104 @@ def cell_harmnumber(ring) -> int:
 
 
 def is_6d(ring) -> bool:
-    """Cavities active or radiating passes present (lattice_object.py)."""
-    for e in ring:
-        pm = getattr(e, "PassMethod", "")
-        if pm.endswith("RadPass") or pm.endswith("CavityPass"):
-            return True
-    return False
+    """Cavities active or radiating passes present."""
+    return any(getattr(e, "PassMethod", "").endswith(("RadPass", "CavityPass")) for e in ring)
+
+
+def _signature(elem):
+    """Hashable description of an element's pass and numeric attributes."""
+    items = []
+    for k, v in sorted(vars(elem).items()):
+        if isinstance(v, np.ndarray):
+            items.append((k, v.shape, v.tobytes()))
+        elif isinstance(v, (int, float, str, bool, np.number)):
+            items.append((k, v))
+    return tuple(items)
 
 
 def get_energy_loss(ring, method=ELossMethod.INTEGRAL) -> float:
-    """Energy loss per turn [eV] (energy_loss.py:31-102); the integral
-    counts dipoles only."""
+    """Energy lost per turn [eV] (energy_loss.py:31-102).
+
+    INTEGRAL: Cgamma E^4 / (2 pi) times the second radiation integral of
+    the dipoles (angle^2 / length). TRACKING: the delta a particle starting
+    at zero loses in each radiating element, summed; identical elements are
+    passed once and counted as often as they occur."""
+    periods = getattr(ring, "periodicity", 1)
     if method is ELossMethod.INTEGRAL:
         i2 = sum(e.BendingAngle ** 2 / e.Length for e in ring
                  if isinstance(e, Dipole) and e.Length > 0)
-        coef = CGAMMA / 2.0 / math.pi * ring.energy ** 4
-        return getattr(ring, "periodicity", 1) * coef * i2
+        return periods * CGAMMA / (2.0 * math.pi) * ring.energy ** 4 * i2
     if method is ELossMethod.TRACKING:
-        delta = 0.0
+        counts, first = {}, {}
         for e in ring:
             if getattr(e, "PassMethod", "").endswith("RadPass"):
-                r = np.zeros((6, 1), order="F")
-                _epass(e, r, energy=ring.energy, particle=ring.particle)
-                delta += r[4, 0]
-        return -delta * ring.energy * getattr(ring, "periodicity", 1)
+                key = _signature(e)
+                counts[key] = counts.get(key, 0) + 1
+                first.setdefault(key, e)
+        lost = 0.0
+        for key, e in first.items():
+            r = np.zeros((6, 1), order="F")
+            _epass(e, r, energy=ring.energy, particle=ring.particle)
+            lost -= counts[key] * r[4, 0]
+        return periods * lost * ring.energy
     raise AtError(f"Invalid method: {method}")
 
 
 def get_timelag_fromU0(ring, *, method=ELossMethod.TRACKING, cavpts=None):
-    """TimeLag that makes the synchronous phase zero, for cavities of one
-    frequency and TimeLag (energy_loss.py:105-206, single-frequency branch)."""
+    """TimeLag of the cavities that puts the synchronous phase at zero, and
+    its difference from the present TimeLag (energy_loss.py:105-206, for
+    cavities of one frequency and one TimeLag: ct_s = c / (2 pi f_rf) *
+    asin(U0 / V_rf))."""
     cav = _cavities(ring) if cavpts is None else [ring[i] for i in _refs(ring, cavpts)]
-    u0 = get_energy_loss(ring, method=method) / getattr(ring, "periodicity", 1)
-    freq = np.unique([c.Frequency for c in cav])
-    tl0 = np.unique([getattr(c, "TimeLag", 0.0) for c in cav])
-    if freq.size > 1 or tl0.size > 1:
+    freqs = {float(c.Frequency) for c in cav}
+    lags = {float(getattr(c, "TimeLag", 0.0)) for c in cav}
+    if len(freqs) > 1 or len(lags) > 1:
         raise AtError("cavities of several frequencies or time lags are not supported")
-    vrf = float(np.sum([c.Voltage for c in cav]))
+    periods = getattr(ring, "periodicity", 1)
+    u0 = get_energy_loss(ring, method=method) / periods
+    vrf = float(sum(c.Voltage for c in cav))
     if u0 > vrf:
-        per = getattr(ring, "periodicity", 1)
-        raise AtError(f"The RF voltage ({per * vrf:.3e} eV) is lower than "
-                      f"the radiation losses ({per * u0:.3e} eV).")
-    timelag = CLIGHT / (2 * math.pi * float(freq[0])) * math.asin(u0 / vrf)
-    return timelag * np.ones(len(cav)), timelag - float(tl0[0])
-
-
-def _orbit6(ring, cavpts=None, guess=None, keep_lattice=False, **kwargs):
-    """6D closed orbit (orbit.py:307-368)."""
-    conv, maxit, xy, dps = _options(kwargs)
-    method = kwargs.pop("method", ELossMethod.TRACKING)
-    if kwargs:
-        raise AtError(f"Unexpected keywords for orbit6: {', '.join(kwargs)}")
-    l0 = cell_length(ring)
-    f_rf = get_rf_frequency(ring)
-    harm = round(f_rf * l0 / beta(ring) / CLIGHT)
-    if guess is None:
-        ref_in = np.zeros(6)
-        _, dt = get_timelag_fromU0(ring, method=method, cavpts=cavpts)
-        ref_in[5] = -dt
-    else:
-        ref_in = np.array(guess, dtype=float)
-    theta = np.zeros(6)
-    theta[5] = beta(ring) * CLIGHT * harm / f_rf - l0
-    scaling = xy * np.array([1.0, 1.0, 1.0, 1.0, 0.0, 0.0]) + dps * np.array(
-        [0.0, 0.0, 0.0, 0.0, 1.0, 1.0])
-    delta = np.diag(scaling)
-    idx = np.arange(6)
-    return _newton(ring, ref_in, delta, idx, idx, np.eye(6),
-                   lambda out, rin: out - rin - theta, keep_lattice, conv, maxit)
-
-
-def find_orbit6(ring, refpts=None, *, orbit=None, keep_lattice=False, **kwargs):
-    """6D closed orbit (orbit.py:371-458)."""
-    if not _cavities(ring):
-        raise AtError("No cavity found in the lattice")
-    if orbit is None:
-        orbit = _orbit6(ring, keep_lattice=keep_lattice, **kwargs)
-        keep_lattice = True
-    return orbit, _propagate(ring, orbit, refpts, keep_lattice)
-
-
-def find_orbit(ring, refpts=None, **kwargs):
-    """find_orbit6 for 6D lattices, find_sync_orbit when dct or df is given,
-    find_orbit4 otherwise (orbit.py:461-519)."""
-    if is_6d(ring):
-        for k in ("dp", "dct", "df"):
-            if kwargs.get(k) is not None:
-                raise AtError(f"{k} is not supported on a 6D lattice")
-            kwargs.pop(k, None)
-        return find_orbit6(ring, refpts=refpts, **kwargs)
-    if kwargs.get("dct") is not None or kwargs.get("df") is not None:
-        return find_sync_orbit(ring, refpts=refpts, **kwargs)
-    return find_orbit4(ring, refpts=refpts, **kwargs)
+        raise AtError(f"The RF voltage ({periods * vrf:.3e} eV) is lower than "
+                      f"the radiation losses ({periods * u0:.3e} eV).")
+    lag = CLIGHT / (2.0 * math.pi * freqs.pop()) * math.asin(u0 / vrf)
+    return np.full(len(cav), lag), lag - lags.pop()
 
 
 # --- transfer matrices --------------------------------------------------------
 
-_JMT = np.array([[0.0, 1.0], [-1.0, 0.0]])
-_JMT4 = np.kron(np.eye(2), _JMT)
+_J4 = np.kron(np.eye(2), np.array([[0.0, 1.0], [-1.0, 0.0]]))
+
+
+def _symplectic_inverse(m: np.ndarray) -> np.ndarray:
+    """Inverse of a symplectic 4x4 matrix, J^T m^T J."""
+    return _J4.T @ m.T @ _J4
 
 
 def find_m44(ring, dp=None, refpts=None, dct=None, df=None, orbit=None,
              keep_lattice=False, **kwargs):
-    """4x4 one-turn matrix by central differences (matrix.py:19-102)."""
+    """4x4 one-turn matrix at the closed orbit (matrix.py:19-102), and the
+    4x4 matrices from the entrance to each refpt; with ``full`` those are
+    turned into one-turn matrices at the refpts (m M44 m^-1)."""
     xy = kwargs.pop("XYStep", DConstant.XYStep)
     full = kwargs.pop("full", False)
     if orbit is None:
         orbit, _ = find_orbit4(ring, dp=dp, dct=dct, df=df, keep_lattice=keep_lattice, XYStep=xy)
         keep_lattice = True
-    scaling = xy * np.ones(4)
-    dg = np.concatenate((0.5 * np.diag(scaling), np.zeros((2, 4))))
-    dmat = np.concatenate((dg, -dg), axis=1)
-    in_mat = np.asfortranarray(orbit.reshape(6, 1) + dmat)
+    steps = xy * _TRANSVERSE
     refs = _refs(ring, refpts)
-    out = _lpass(ring, in_mat, refs, keep_lattice)
-    out_mat = np.rollaxis(np.squeeze(out, axis=3), -1)
-    m44 = (in_mat[:4, :4] - in_mat[:4, 4:]) / scaling
-    if refs.size > 0:
-        mstack = (out_mat[:, :4, :4] - out_mat[:, :4, 4:]) / scaling
-        if full:
-            mstack = np.stack([m.dot(m44.dot(_JMT4.T.dot(m.T.dot(_JMT4)))) for m in mstack])
-    else:
-        mstack = np.empty((0, 4, 4))
+    _, jac, copies = _probe(ring, orbit, steps, refs, keep_lattice)
+    m44 = jac[:4]
+    if refs.size == 0:
+        return m44, np.empty((0, 4, 4))
+    mstack = _differences(copies, steps, np.arange(4))[:, :4]
+    if full:
+        mstack = np.stack([m @ m44 @ _symplectic_inverse(m) for m in mstack])
     return m44, mstack
 
 
 def find_m66(ring, refpts=None, orbit=None, keep_lattice=False, **kwargs):
-    """6x6 one-turn matrix by central differences (matrix.py:105-168)."""
+    """6x6 one-turn matrix at the closed orbit and the matrices from the
+    entrance to each refpt (matrix.py:105-168)."""
     xy = kwargs.pop("XYStep", DConstant.XYStep)
     dps = kwargs.pop("DPStep", DConstant.DPStep)
     if orbit is None:
@@ -359,30 +422,19 @@ def find_m66(ring, refpts=None, orbit=None, keep_lattice=False, **kwargs):
         else:
             orbit, _ = find_orbit4(ring, keep_lattice=keep_lattice, XYStep=xy, **kwargs)
         keep_lattice = True
-    scaling = xy * np.array([1.0, 1.0, 1.0, 1.0, 0.0, 0.0]) + dps * np.array(
-        [0.0, 0.0, 0.0, 0.0, 1.0, 1.0])
-    dg = 0.5 * np.diag(scaling)
-    dmat = np.concatenate((dg, -dg), axis=1)
-    in_mat = np.asfortranarray(orbit.reshape(6, 1) + dmat)
+    steps = xy * _TRANSVERSE + dps * (1.0 - _TRANSVERSE)
     refs = _refs(ring, refpts)
-    out = _lpass(ring, in_mat, refs, keep_lattice)
-    out_mat = np.rollaxis(np.squeeze(out, axis=3), -1)
-    m66 = (in_mat[:, :6] - in_mat[:, 6:]) / scaling
-    if refs.size > 0:
-        mstack = (out_mat[:, :, :6] - out_mat[:, :, 6:]) / scaling
-    else:
-        mstack = np.empty((0, 6, 6))
-    return m66, mstack
+    _, m66, copies = _probe(ring, orbit, steps, refs, keep_lattice)
+    if refs.size == 0:
+        return m66, np.empty((0, 6, 6))
+    return m66, _differences(copies, steps, np.arange(6))
 
 
 def find_elem_m66(elem, orbit=None, **kwargs):
-    """6x6 matrix of one element (matrix.py:171-205)."""
+    """6x6 matrix of one element around ``orbit`` (default zero) by central
+    differences with step XYStep on every coordinate (matrix.py:171-205)."""
     xy = kwargs.pop("XYStep", DConstant.XYStep)
-    if orbit is None:
-        orbit = np.zeros(6)
-    scaling = xy * np.ones(6)
-    dg = 0.5 * np.diag(scaling)
-    dmat = np.concatenate((dg, -dg), axis=1)
-    in_mat = np.asfortranarray(orbit.reshape(6, 1) + dmat)
-    _epass(elem, in_mat, **kwargs)
-    return (in_mat[:, :6] - in_mat[:, 6:]) / scaling
+    steps = xy * np.ones(6)
+    batch, varied = _probe_batch(np.zeros(6) if orbit is None else orbit, steps)
+    _epass(elem, batch, **kwargs)
+    return _differences(batch, steps, varied)

@@ -18,14 +18,14 @@ fixtures next to this script:
                  with the inputs of pyat/test/test_atpass.py:157-171, and
                  8 particles x 5 turns with refpts;
   large_ring_1000.npz  the large ring, 2048 particles of the sigma x 10
-                 beam, 1000 turns with losses, and the chaotic mask: the
-                 particles whose loss record or final coordinates (by more
-                 than 1e-9) change when the inputs are scaled by 1 + eps,
-                 eps in PERTURBATIONS (up to 1e-13 relative: the size of the
-                 rounding differences that FMA contraction accumulates along
-                 a regular orbit in a few hundred turns); written only when
-                 asked for by name, `make_golden.py large_ring_1000` (about
-                 an hour on 8 cores).
+                 beam, 1000 turns with losses, and one mask per input
+                 perturbation: the particles whose loss record or final
+                 coordinates (by more than 1e-9) change when the inputs are
+                 scaled by 1 + eps, eps in PERTURBATIONS (from the rounding
+                 scale, 4e-16, up to 1e-13 relative); `rounding_layer` is the
+                 union of the masks with |eps| <= 2e-15, `chaotic` the union
+                 of all of them. Written only when asked for by name,
+                 `make_golden.py large_ring_1000` (about an hour on 8 cores).
 
 Only this script runs reference code; the fixtures are data.
 """
@@ -105,16 +105,21 @@ def make_large_ring_1000(nthreads: int):
     r = r0.copy(order="F")
     _, lm = po.atpass(ring, r, 1000, losses=True, impl="ref", nthreads=nthreads)
     rec = _records(lm)
-    chaotic = np.zeros(r0.shape[1], bool)
-    for eps in PERTURBATIONS:
+    # one mask per perturbation: the particles whose loss record, or final
+    # coordinates by more than 1e-9, change when the input is scaled by 1 + eps
+    masks = np.zeros((len(PERTURBATIONS), r0.shape[1]), bool)
+    for k, eps in enumerate(PERTURBATIONS):
         rp = np.asfortranarray(r0 * (1.0 + eps))
         _, lp = po.atpass(ring, rp, 1000, losses=True, impl="ref", nthreads=nthreads)
         surv = ~np.asarray(lm["islost"]) & ~np.asarray(lp["islost"])
-        chaotic |= np.any(_records(lp) != rec, axis=0)
-        chaotic[surv] |= np.max(np.abs(r[:, surv] - rp[:, surv]), axis=0) > 1e-9
-        print(f"perturbation {eps:g}: chaotic so far {int(chaotic.sum())}", flush=True)
+        masks[k] = np.any(_records(lp) != rec, axis=0)
+        masks[k, surv] |= np.max(np.abs(r[:, surv] - rp[:, surv]), axis=0) > 1e-9
+        print(f"perturbation {eps:g}: {int(masks[k].sum())} particles move, "
+              f"union so far {int(masks[:k + 1].any(axis=0).sum())}", flush=True)
+    rounding = np.abs(np.array(PERTURBATIONS)) <= 2e-15
     np.savez_compressed(os.path.join(HERE, "large_ring_1000.npz"), rin=r0, rout_final=r,
-                        nturns=1000, chaotic=chaotic, perturbations=np.array(PERTURBATIONS),
+                        nturns=1000, perturbations=np.array(PERTURBATIONS), masks=masks,
+                        chaotic=masks.any(axis=0), rounding_layer=masks[rounding].any(axis=0),
                         **{f"loss_{k}": np.asarray(v) for k, v in lm.items()})
 
 

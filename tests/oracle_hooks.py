@@ -5,7 +5,7 @@ run the same drivers on the engine."""
 
 import numpy as np
 
-from at_amd import acceptance, envelope, physics
+from at_amd import acceptance, physics
 from oracle import pyoracle as po
 
 
@@ -35,28 +35,8 @@ def oracle_project(ring, parts, obs, **kwargs):
     po.atpass(ring, parts, 1, start_elem=int(obs), energy=e, particle=part)
 
 
-def oracle_diffmats(ring, orbits):
-    e, _ = _ep(ring)
-    return np.stack([po.diffusion_ref(el, orbits[:, i], e) for i, el in enumerate(ring)])
-
-
-def oracle_elem_m66s(ring, orbits, xystep):
-    e, part = _ep(ring)
-    scaling = xystep * np.ones(6)
-    dg = 0.5 * np.diag(scaling)
-    dmat = np.concatenate((dg, -dg), axis=1)
-    out = np.empty((len(ring), 6, 6))
-    for i, el in enumerate(ring):
-        r = np.asfortranarray(orbits[:, i].reshape(6, 1) + dmat)
-        po.elempass(el, r, energy=e, particle=part)
-        out[i] = (r[:, :6] - r[:, 6:]) / scaling
-    return out
-
-
 def install(monkeypatch):
     monkeypatch.setattr(physics, "_lpass", oracle_lpass)
     monkeypatch.setattr(physics, "_epass", oracle_epass)
     monkeypatch.setattr(acceptance, "_survived", oracle_survived)
     monkeypatch.setattr(acceptance, "_project", oracle_project)
-    monkeypatch.setattr(envelope, "_diffmats", oracle_diffmats)
-    monkeypatch.setattr(envelope, "_elem_m66s", oracle_elem_m66s)
