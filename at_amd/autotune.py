@@ -13,7 +13,7 @@ import json
 import os
 import threading
 
-from . import engine
+from . import device, engine
 
 CANDIDATES = (64, 128, 256, 512)
 _CACHE_FILE = os.environ.get("ATX_TUNE_CACHE",
@@ -50,14 +50,13 @@ def _store(d: dict) -> None:
 
 
 def tune_batch(lattices, particles, *, energy: float, rest_energy: float, probe_turns: int = 3,
-               candidates=CANDIDATES, strict: bool = False, use_cache: bool = True) -> dict:
+               candidates=CANDIDATES, strict: bool = False, use_cache: bool = True,
+               repack: bool = False) -> dict:
     """Pick the workgroup size for one batched launch.
 
-    lattices: DeviceLattice per instance; particles: torch FP64 tensors
-    (N_i, 6) on the device (left untouched: probes track clones).
+    lattices: DeviceLattice per instance; particles: FP64 device buffers
+    (N_i, 6) (left untouched: probes track clones).
     Returns {"block_size": b, "ms": {b: best probe time}}."""
-    import torch
-
     from .batch import track_batch
 
     key = _key(lattices, particles, strict)
@@ -69,20 +68,20 @@ def tune_batch(lattices, particles, *, energy: float, rest_energy: float, probe_
             if key in disk:
                 _mem[key] = disk[key]
                 return disk[key]
-    stream = torch.cuda.current_stream()
+    stream = device.Stream()
+    e0, e1 = device.Event(), device.Event()
     times = {}
     for b in candidates:
         best = None
         for _ in range(2):
             scratch = [p.clone() for p in particles]
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
+            device.synchronize()
             e0.record(stream)
             track_batch(lattices, scratch, probe_turns, energy=energy, rest_energy=rest_energy,
-                        losses=False, block_size=b, strict=strict)
+                        losses=False, block_size=b, strict=strict, stream=stream)
             e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1)
             best = ms if best is None else min(best, ms)
         times[b] = best
     choice = {"block_size": int(min(times, key=times.get)),

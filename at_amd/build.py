@@ -2,9 +2,10 @@
 
     python -m at_amd.build          # builds at_amd/libatx.so
 
-The kernel source is compiled twice (strict: reference operation order and
-no FP contraction; fast: FMA contraction) and linked with the host runtime
-into one library that exports the C ABI of include/atx.h.
+The kernel source is compiled once per build (strict: reference operation
+order and no FP contraction; fast: FMA contraction) and launch shape, the
+objects in parallel, and linked with the host runtime into one library that
+exports the C ABI of include/atx.h.
 """
 
 from __future__ import annotations
@@ -13,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -39,25 +41,43 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+# Launch shapes with a kernel of their own: particles per lane x slices per
+# loop iteration (include/atx.h atx_launch_shape). Two particles with four
+# slices per iteration would need more than a lane's 256 registers.
+SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2)]
+
+
+def _jobs() -> int:
+    env = os.environ.get("MAX_JOBS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
+
+
 def build(verbose_isa: bool = False) -> str:
+    """Compile every object in parallel and link libatx.so. The strict build
+    keeps the reference operation order (no contraction); the fast build
+    contracts a*b+c within an expression only (-ffp-contract=on), so that the
+    kernels of all launch shapes round identically."""
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
-    objs = []
-    for name, strict, contract in (("strict", 1, "off"), ("fast", 0, "fast")):
-        obj = os.path.join(BUILD, f"atx_kernel_{name}.o")
-        cmd = [cc, "-c", *COMMON, f"-DATX_STRICT={strict}", f"-ffp-contract={contract}",
-               os.path.join(CSRC, "atx_kernel.hip"), "-o", obj]
-        if verbose_isa:
-            cmd += ["-Rpass-analysis=kernel-resource-usage"]
-        _run(cmd)
-        objs.append(obj)
-    adj_obj = os.path.join(BUILD, "atx_adjoint.o")
-    _run([cc, "-c", *COMMON, "-ffp-contract=fast", os.path.join(CSRC, "atx_adjoint.hip"), "-o",
-          adj_obj] + (["-Rpass-analysis=kernel-resource-usage"] if verbose_isa else []))
-    objs.append(adj_obj)
-    host_obj = os.path.join(BUILD, "atx_host.o")
-    _run([cc, "-c", *COMMON, "-ffp-contract=off", os.path.join(CSRC, "atx_host.cpp"), "-o", host_obj])
-    objs.append(host_obj)
+    isa = ["-Rpass-analysis=kernel-resource-usage"] if verbose_isa else []
+    cmds = []
+    for name, strict, contract in (("strict", 1, "off"), ("fast", 0, "on")):
+        for ppt, unroll in SHAPES:
+            obj = os.path.join(BUILD, f"atx_kernel_{name}_p{ppt}_u{unroll}.o")
+            cmds.append([cc, "-c", *COMMON, f"-DATX_STRICT={strict}", f"-DATX_PPT={ppt}",
+                         f"-DATX_UNROLL={unroll}", f"-ffp-contract={contract}",
+                         os.path.join(CSRC, "atx_kernel.hip"), "-o", obj] + isa)
+    cmds.append([cc, "-c", *COMMON, "-ffp-contract=fast", os.path.join(CSRC, "atx_adjoint.hip"),
+                 "-o", os.path.join(BUILD, "atx_adjoint.o")] + isa)
+    cmds.append([cc, "-c", *COMMON, "-ffp-contract=off", os.path.join(CSRC, "atx_host.cpp"), "-o",
+                 os.path.join(BUILD, "atx_host.o")])
+    objs = [c[c.index("-o") + 1] for c in cmds]
+    # the longest compilations first
+    order = sorted(range(len(cmds)), key=lambda i: ("_p2_" not in objs[i], objs[i]))
+    with ThreadPoolExecutor(max_workers=_jobs()) as pool:
+        for f in [pool.submit(_run, cmds[i]) for i in order]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -79,7 +79,8 @@ enum Slot : int32_t {
   S_RK1 = 19, S_RK2 = 20,                   // S_RADC * S_K1, S_RADC * S_K2 (fast build)
   S_ELEN = 21,                              // Length (exact multipoles)
   S_DIFC = 22,                              // DIF_CONST * gamma^5 (diffusion matrices)
-  S_NSCALAR = 24,
+  S_HL1 = 23, S_HL2 = 24,                   // S_L1 / 2, S_L2 / 2 (fast radiating drifts)
+  S_NSCALAR = 26,
   // RF cavity
   S_RF_HALF = 0, S_RF_NV = 1, S_RF_TWOPIF = 2, S_RF_LAG = 3, S_RF_PHILAG = 4, S_RF_HFT0 = 5,
   // corrector

@@ -19,13 +19,25 @@
 #include "../../include/atx.h"
 #include "atx_device.h"
 
-extern "C" hipError_t atx_launch_track_strict(const atx::Instance*, int, int, int, int, const int32_t*,
-                                              hipStream_t);
-extern "C" hipError_t atx_launch_track_fast(const atx::Instance*, int, int, int, int, const int32_t*,
-                                            hipStream_t);
+// Tracking kernels, one per build and launch shape (particles per lane x
+// slices per loop iteration), compiled from atx_kernel.hip as separate objects.
+#define ATX_TRACK_DECL(name)                                                                  \
+  extern "C" hipError_t name(const atx::Instance*, int, int, int, int, const int32_t*, hipStream_t);
+ATX_TRACK_DECL(atx_launch_track_strict_p1_u1)
+ATX_TRACK_DECL(atx_launch_track_strict_p1_u2)
+ATX_TRACK_DECL(atx_launch_track_strict_p1_u4)
+ATX_TRACK_DECL(atx_launch_track_strict_p2_u1)
+ATX_TRACK_DECL(atx_launch_track_strict_p2_u2)
+ATX_TRACK_DECL(atx_launch_track_fast_p1_u1)
+ATX_TRACK_DECL(atx_launch_track_fast_p1_u2)
+ATX_TRACK_DECL(atx_launch_track_fast_p1_u4)
+ATX_TRACK_DECL(atx_launch_track_fast_p2_u1)
+ATX_TRACK_DECL(atx_launch_track_fast_p2_u2)
+#undef ATX_TRACK_DECL
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
 extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
-extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int, int32_t*, hipStream_t);
+extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int, int, int32_t*,
+                                          hipStream_t);
 extern "C" hipError_t atx_launch_adjoint(const atx::AdjArgs*, int32_t, hipStream_t);
 
 namespace atx {
@@ -188,6 +200,7 @@ struct atx_lattice {
   size_t d_elems_n = 0, d_pool_n = 0;
   int64_t turn_counter = 0;
   int integrator = 4;
+  atx_launch_shape shape{0, 0, 0, 0};  // atx_track / atx_elempass launches (0: default)
   // host copies of per-element compile results used by the adjoint
   std::vector<int32_t> h_r1inv, h_r2inv, h_order, h_kind;
   std::vector<std::string> h_adjerr;
@@ -463,6 +476,8 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       const double sl = len / (double)nslice;
       sc[S_L1] = sl * kDrift1;
       sc[S_L2] = sl * kDrift2;
+      sc[S_HL1] = 0.5 * sc[S_L1];
+      sc[S_HL2] = 0.5 * sc[S_L2];
       sc[S_K1] = sl * kKick1;
       sc[S_K2] = sl * kKick2;
       sc[S_ELEN] = len;
@@ -666,6 +681,40 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
 }
 
 constexpr int kBlock = 256;
+constexpr int kPpt = 1;
+constexpr int kUnroll = 2;
+
+typedef hipError_t (*TrackLaunch)(const Instance*, int, int, int, int, const int32_t*, hipStream_t);
+
+// The tracking kernel of a build and a (valid) launch shape.
+TrackLaunch track_launcher(bool strict, const atx_launch_shape& sh) {
+  static const TrackLaunch table[2][2][3] = {
+      {{atx_launch_track_fast_p1_u1, atx_launch_track_fast_p1_u2, atx_launch_track_fast_p1_u4},
+       {atx_launch_track_fast_p2_u1, atx_launch_track_fast_p2_u2, nullptr}},
+      {{atx_launch_track_strict_p1_u1, atx_launch_track_strict_p1_u2, atx_launch_track_strict_p1_u4},
+       {atx_launch_track_strict_p2_u1, atx_launch_track_strict_p2_u2, nullptr}}};
+  const int u = sh.unroll == 4 ? 2 : sh.unroll - 1;
+  return table[strict ? 1 : 0][sh.ppt - 1][u];
+}
+
+// Launch shape with the defaults filled in, or an error for an invalid one.
+int resolve_shape(const atx_launch_shape* in, atx_launch_shape& out) {
+  out = atx_launch_shape{kBlock, kPpt, kUnroll, 0};
+  if (in) {
+    if (in->block) out.block = in->block;
+    if (in->ppt) out.ppt = in->ppt;
+    if (in->unroll) out.unroll = in->unroll;
+  }
+  if (out.block % 64 || out.block < 64 || out.block > 512)
+    return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
+  if (out.ppt != 1 && out.ppt != 2) return fail(ATX_E_VALUE, "particles per lane must be 1 or 2");
+  if (out.unroll != 1 && out.unroll != 2 && out.unroll != 4)
+    return fail(ATX_E_VALUE, "slice unroll must be 1, 2 or 4");
+  // four slices of two particles need more than the 256 registers of a lane
+  if (out.ppt == 2 && out.unroll == 4)
+    return fail(ATX_E_VALUE, "two particles per lane take a slice unroll of 1 or 2");
+  return ATX_OK;
+}
 
 // Pinned staging for the launch tables (instance tables, refpt slot tables,
 // adjoint tables). hipMemcpyAsync from pageable memory returns only once the
@@ -721,13 +770,13 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t st
 // slots[i] is the refpt slot table of instance i; instances that share a
 // table (same pointer) share its device copy.
 int launch_instances(std::vector<Instance>& insts, const std::vector<const std::vector<int32_t>*>& slots,
-                     int block, hipStream_t stream, bool strict, bool ext) {
-  if (block <= 0) block = kBlock;
-  if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
+                     const atx_launch_shape& shape, hipStream_t stream, bool strict, bool ext) {
+  const int block = shape.block;
+  const int per_wg = block * shape.ppt;  // particle slots per workgroup
   int nblocks = 0;
   for (auto& in : insts) {
     in.block_begin = nblocks;
-    nblocks += (int)((in.npart + block - 1) / block);
+    nblocks += (int)((in.npart + per_wg - 1) / per_wg);
   }
   if (nblocks == 0) return ATX_OK;
   size_t bytes = insts.size() * sizeof(Instance);
@@ -754,10 +803,8 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   hipError_t ce = upload_async(d_scratch, host.data(), bytes, stream);
   hipError_t le = hipSuccess;
   if (ce == hipSuccess)
-    le = strict ? atx_launch_track_strict((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
-                                          ext ? 1 : 0, nullptr, stream)
-                : atx_launch_track_fast((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
-                                        ext ? 1 : 0, nullptr, stream);
+    le = track_launcher(strict, shape)((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
+                                       ext ? 1 : 0, nullptr, stream);
   hipError_t fe = hipFreeAsync(d_scratch, stream);
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("upload: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
@@ -803,9 +850,10 @@ std::vector<int> repack_bounds(int nturns) {
 // private stream into one graph, which runs on `stream`. Each particle's
 // arithmetic is unchanged, so results equal those of launch_instances.
 int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::vector<int32_t>*>& slots,
-                    int block, hipStream_t stream, bool strict, bool ext) {
-  if (block <= 0) block = kBlock;
-  if (block % 64 || block > 512) return fail(ATX_E_VALUE, "block size must be a multiple of 64 in [64, 512]");
+                    const atx_launch_shape& shape, hipStream_t stream, bool strict, bool ext) {
+  const int block = shape.block, ppt = shape.ppt;
+  const int per_wg = block * ppt;  // particle slots per workgroup
+  const int fine = 64 * ppt;       // ... when the survivors run one wave per workgroup
   reap_graphs();
   const size_t ninst = insts.size();
   int nblocks = 0;
@@ -814,8 +862,8 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   int64_t nidx = 0;
   for (auto& in : insts) {
     in.block_begin = nblocks;
-    nblocks += (int)((in.npart + block - 1) / block);
-    nblocks_fine += (int)((in.npart + 63) / 64);
+    nblocks += (int)((in.npart + per_wg - 1) / per_wg);
+    nblocks_fine += (int)((in.npart + fine - 1) / fine);
     maxt = std::max(maxt, (int)in.nturns);
     nidx += in.npart;
   }
@@ -851,7 +899,7 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     ncu = 0;
-  const int fine_limit = 2 * 64 * 4 * ncu;
+  const int fine_limit = 2 * fine * 4 * ncu;
   char* d = nullptr;
   HIPCHK(hipMallocAsync((void**)&d, bytes, stream));
   std::vector<char> host(upload);
@@ -888,12 +936,11 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
     for (int k = 0; k < nblk && le == hipSuccess; k++) {
       const Instance* t = (const Instance*)(d + (size_t)k * tab);
       // after the first block the survivors' workgroups lead the grid
-      if (k > 0) le = atx_launch_live_map(t, (int)ninst, block, fine_limit, live, cap);
+      if (k > 0) le = atx_launch_live_map(t, (int)ninst, per_wg, fine, fine_limit, live, cap);
       if (le != hipSuccess) break;
       const int32_t* lb = k > 0 ? live : nullptr;
       const int grid = k > 0 ? nblocks_fine : nblocks;
-      le = strict ? atx_launch_track_strict(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap)
-                  : atx_launch_track_fast(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap);
+      le = track_launcher(strict, shape)(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap);
       if (le == hipSuccess && k + 1 < nblk)
         le = atx_launch_repack((const Instance*)(d + (size_t)(k + 1) * tab), (int)ninst, k == 0 ? 1 : 0, cap);
     }
@@ -972,7 +1019,19 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
                            d_rout, d_lf, d_lt, d_le, d_lc, losses, elempass, slots[0].data(), insts[0]);
   if (rc) return rc;
   if (np == 0 || nturns <= 0) return ATX_OK;
-  return launch_instances(insts, {&slots[0]}, kBlock, stream, strict, lat->has_ext);
+  atx_launch_shape shape;
+  rc = resolve_shape(&lat->shape, shape);
+  if (rc) return rc;
+  // Without a shape set on the lattice, beams of two or more 512-lane
+  // workgroups per CU run in 512-lane workgroups: survivors are packed
+  // within a workgroup, and larger ones pack more of them (2 % faster on the
+  // large ring at 262,144 particles than 256 lanes).
+  int dev = 0, ncu = 0;
+  if (lat->shape.block == 0 && hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      (int64_t)np >= (int64_t)2 * 512 * shape.ppt * ncu)
+    shape.block = 512;
+  return launch_instances(insts, {&slots[0]}, shape, stream, strict, lat->has_ext);
 }
 
 int check_partial(int nelem, int nturns, int& start, int& end) {
@@ -1216,11 +1275,13 @@ int atx_shutdown(void) {
 }
 
 int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns, int32_t turn,
-                    double energy, double rest_energy, int32_t losses, int32_t block_size,
-                    void* stream_, uint32_t flags) {
+                    double energy, double rest_energy, int32_t losses,
+                    const atx_launch_shape* shape_in, void* stream_, uint32_t flags) {
   g_err.clear();
   if (nitems < 0 || (nitems > 0 && !items)) return fail(ATX_E_VALUE, "bad item list");
   if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  atx_launch_shape shape;
+  if (int rc = resolve_shape(shape_in, shape)) return rc;
   hipStream_t stream = (hipStream_t)stream_;
   const bool strict = (flags & ATX_STRICT) != 0;
   const bool soa = (flags & ATX_SOA) != 0;
@@ -1295,8 +1356,17 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
     sorted_slots.push_back(slots[i]);
   }
   if (flags & ATX_REPACK)
-    return launch_repacked(sorted_insts, sorted_slots, block_size, stream, strict, ext);
-  return launch_instances(sorted_insts, sorted_slots, block_size, stream, strict, ext);
+    return launch_repacked(sorted_insts, sorted_slots, shape, stream, strict, ext);
+  return launch_instances(sorted_insts, sorted_slots, shape, stream, strict, ext);
+}
+
+int atx_lattice_set_launch_shape(atx_lattice* lat, const atx_launch_shape* shape) {
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  atx_launch_shape s;
+  if (int rc = resolve_shape(shape, s)) return rc;
+  std::lock_guard<std::mutex> g(lat->mu);
+  lat->shape = shape ? *shape : atx_launch_shape{0, 0, 0, 0};
+  return ATX_OK;
 }
 
 int atx_element_set_array_at(atx_lattice* lat, int32_t index, const char* name, const double* data,
@@ -1540,6 +1610,114 @@ int atx_device_info(int device, char* name, int name_len, char* arch, int arch_l
 int atx_set_device(int device) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(ATX_E_NODEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return ATX_OK;
+}
+
+int atx_get_device(int* device) {
+  if (!device) return fail(ATX_E_VALUE, "NULL argument");
+  HIPCHK(hipGetDevice(device));
+  return ATX_OK;
+}
+
+// --- Device memory, pinned host memory, streams and events -----------------
+
+int atx_device_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(ATX_E_VALUE, "NULL argument");
+  *ptr = nullptr;
+  if (bytes == 0) return ATX_OK;
+  HIPCHK(hipMalloc(ptr, bytes));
+  return ATX_OK;
+}
+
+int atx_device_free(void* ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return ATX_OK;
+}
+
+int atx_host_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(ATX_E_VALUE, "NULL argument");
+  *ptr = nullptr;
+  if (bytes == 0) return ATX_OK;
+  HIPCHK(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return ATX_OK;
+}
+
+int atx_host_free(void* ptr) {
+  if (ptr) HIPCHK(hipHostFree(ptr));
+  return ATX_OK;
+}
+
+int atx_memcpy(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (bytes == 0) return ATX_OK;
+  if (!dst || !src) return fail(ATX_E_VALUE, "NULL argument");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, static_cast<hipStream_t>(stream)));
+  return ATX_OK;
+}
+
+int atx_stream_create(void** stream) {
+  if (!stream) return fail(ATX_E_VALUE, "NULL argument");
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return ATX_OK;
+}
+
+int atx_stream_destroy(void* stream) {
+  if (stream) HIPCHK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return ATX_OK;
+}
+
+int atx_stream_synchronize(void* stream) {
+  HIPCHK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ATX_OK;
+}
+
+int atx_stream_wait_event(void* stream, void* event) {
+  if (!event) return fail(ATX_E_VALUE, "NULL event");
+  HIPCHK(hipStreamWaitEvent(static_cast<hipStream_t>(stream), static_cast<hipEvent_t>(event), 0));
+  return ATX_OK;
+}
+
+int atx_event_create(void** event) {
+  if (!event) return fail(ATX_E_VALUE, "NULL argument");
+  hipEvent_t e = nullptr;
+  HIPCHK(hipEventCreate(&e));
+  *event = e;
+  return ATX_OK;
+}
+
+int atx_event_destroy(void* event) {
+  if (event) HIPCHK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return ATX_OK;
+}
+
+int atx_event_record(void* event, void* stream) {
+  if (!event) return fail(ATX_E_VALUE, "NULL event");
+  HIPCHK(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return ATX_OK;
+}
+
+int atx_event_query(void* event, int* done) {
+  if (!event || !done) return fail(ATX_E_VALUE, "NULL argument");
+  hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return ATX_OK;
+  }
+  HIPCHK(e);
+  *done = 1;
+  return ATX_OK;
+}
+
+int atx_event_synchronize(void* event) {
+  if (!event) return fail(ATX_E_VALUE, "NULL event");
+  HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+  return ATX_OK;
+}
+
+int atx_event_elapsed(float* ms, void* start, void* end) {
+  if (!ms || !start || !end) return fail(ATX_E_VALUE, "NULL argument");
+  HIPCHK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
   return ATX_OK;
 }
 

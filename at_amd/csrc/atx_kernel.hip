@@ -25,13 +25,22 @@
 #ifndef ATX_STRICT
 #error "ATX_STRICT must be defined to 0 or 1"
 #endif
+// One object per launch shape (particles per lane ATX_PPT, slices per loop
+// iteration ATX_UNROLL) and build, compiled in parallel; the helper kernels
+// (repack, live map, diffusion matrices) go into the strict 1x1 object.
+#if !defined(ATX_PPT) || !defined(ATX_UNROLL)
+#error "ATX_PPT (1, 2) and ATX_UNROLL (1, 2, 4) must be defined"
+#endif
+#define ATX_AUX (ATX_STRICT && ATX_PPT == 1 && ATX_UNROLL == 1)
+#define ATX_PASTE6(a, b, c, d, e, f) a##b##c##d##e##f
+#define ATX_NAME(build, p, u) ATX_PASTE6(atx_launch_track_, build, _p, p, _u, u)
 
 #if ATX_STRICT
 #define ATX_NS strict_ns
-#define ATX_LAUNCH atx_launch_track_strict
+#define ATX_LAUNCH ATX_NAME(strict, ATX_PPT, ATX_UNROLL)
 #else
 #define ATX_NS fast_ns
-#define ATX_LAUNCH atx_launch_track_fast
+#define ATX_LAUNCH ATX_NAME(fast, ATX_PPT, ATX_UNROLL)
 #endif
 
 namespace atx {
@@ -513,69 +522,246 @@ __device__ __noinline__ R6 slices_ops(R6 r, cdouble* a, cdouble* b, int n, cdoub
   return r;
 }
 
-// StrMPoleSymplectic4Pass.c:57-104 and BndMPoleSymplectic4Pass.c:75-122.
-template <bool BEND, int N>
-__device__ __forceinline__ void mpole(R6& r, cDevElem& h, cdouble* pool) {
+// Particles per lane: the element code below takes the P particles a lane
+// tracks as an array and runs every step of a magnet for all of them before
+// the next step, so the P independent dependency chains interleave in one
+// instruction stream, and the magnet's scalar constants, loop control and
+// branches serve P particles. Per-particle maps that are rare or large stay
+// out of line and run once per particle.
+#if !ATX_STRICT
+// In-magnet drift of the fast build: nlh = nl / (2 (1+delta)) hoisted with
+// nl (delta is constant in a non-radiating magnet).
+__device__ __forceinline__ void fastdrift_h(R6& r, double nl, double nlh) {
+  r.x = __builtin_fma(nl, r.px, r.x);
+  r.y = __builtin_fma(nl, r.py, r.y);
+  const double p2 = __builtin_fma(r.px, r.px, r.py * r.py);
+  r.ct = __builtin_fma(nlh, p2, r.ct);
+}
+
+// Radiating magnets of the fast build carry the transverse momenta as
+// angles x' = px/(1+delta), y' = py/(1+delta) from entry to exit: a drift is
+// then x += L x', ct += (L/2)(x'^2 + y'^2), its x'^2 + y'^2 is the kick's
+// s2 (the kick that follows a drift sees the same angles), and a kick turns
+// px = x'(1+delta') - kl Re, the reference's update, into
+// x' <- x' - kl Re / (1+delta'). pn = 1/(1+delta) as in rad_str_kick.
+struct Ang {
+  double xp, yp, s2, pn;
+};
+
+__device__ __forceinline__ Ang to_angles(const R6& r) {
+  const double pn = fast_rcp(1.0 + r.dp);
+  return Ang{r.px * pn, r.py * pn, 0.0, pn};
+}
+
+__device__ __forceinline__ void from_angles(R6& r, const Ang& g) {
+  const double opd = 1.0 + r.dp;
+  r.px = g.xp * opd;
+  r.py = g.yp * opd;
+}
+
+__device__ __forceinline__ void rad_drift_ang(R6& r, Ang& g, double len, double hlen) {
+  r.x = __builtin_fma(len, g.xp, r.x);
+  r.y = __builtin_fma(len, g.yp, r.y);
+  g.s2 = __builtin_fma(g.xp, g.xp, g.yp * g.yp);
+  r.ct = __builtin_fma(hlen, g.s2, r.ct);
+}
+
+// diff_str_kick.c:19-83 (bdiff == NULL) in angle form; radk = RAD_CONST
+// gamma^3 kl. The radiated factor (1+s2/2)/(1+s2) is the series of
+// rad_str_kick (relative error s2^4/2; a wave with s2 > 1e-4 divides).
+template <int N>
+__device__ __forceinline__ void rad_str_kick_ang(R6& r, Ang& g, cdouble* a, cdouble* b, int n,
+                                                 double blo, double alo, double kl, double radk) {
+  double re, im;
+  field<N>(r, a, b, n, blo, alo, re, im);
+  const double s2 = g.s2;
+  const double opd = 1.0 + r.dp;
+  const double cr = __builtin_fma(im, g.yp, -re * g.xp);
+  const double b2 = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im));
+  double f = __builtin_fma(-0.5 * s2, __builtin_fma(s2, s2, 1.0 - s2), 1.0);
+  const bool wide = s2 > 1e-4;
+  if (__builtin_expect(__ballot(wide) != 0, 0)) {
+    if (wide) f = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);
+  }
+  r.dp = __builtin_fma(-radk, (b2 * f) * (opd * opd), r.dp);
+  const double pn = rcp_after_kick(1.0 + r.dp, g.pn);
+  g.xp = __builtin_fma(-kl * re, pn, g.xp);
+  g.yp = __builtin_fma(kl * im, pn, g.yp);
+  g.pn = pn;
+}
+
+// diff_bnd_kick.c:85-174 (bdiff == NULL) in angle form.
+template <int N>
+__device__ __forceinline__ void rad_bnd_kick_ang(R6& r, Ang& g, cdouble* a, cdouble* b, int n,
+                                                 double blo, double alo, double kl, double irho,
+                                                 double kli, double radk) {
+  double re, im;
+  field<N>(r, a, b, n, blo, alo, re, im);
+  const double x = r.x;
+  const double s2 = g.s2;
+  const double d0 = r.dp;
+  const double opd = 1.0 + d0;
+  const double by = re + irho;
+  const double hx = __builtin_fma(x, irho, 1.0);
+  const double byh = by * hx;
+  const double bxh = im * hx;
+  const double cr = __builtin_fma(im, g.yp, -by * g.xp);
+  const double b2 = __builtin_fma(cr, cr, __builtin_fma(byh, byh, bxh * bxh));
+  const double f = __builtin_fma(0.5, s2, hx) * rcp1(__builtin_fma(hx, hx, s2));
+  r.dp = __builtin_fma(-radk, (b2 * f) * (opd * opd), d0);
+  const double pn = rcp_after_kick(1.0 + r.dp, g.pn);
+  const double u = __builtin_fma(-x, irho, d0);
+  const double v = __builtin_fma(-u, irho, re);
+  g.xp = __builtin_fma(-kl * v, pn, g.xp);
+  g.yp = __builtin_fma(kl * im, pn, g.yp);
+  r.ct = __builtin_fma(kli, x, r.ct);
+  g.pn = pn;
+}
+#endif
+
+// Entry and exit of a thick multipole, per particle: FieldScaling, T1/R1,
+// apertures, bend edge, quadrupole fringe (StrMPoleSymplectic4Pass.c:57-78,
+// 96-104; BndMPoleSymplectic4Pass.c:75-96, 114-122).
+template <bool BEND>
+__device__ __forceinline__ void mpole_entry(R6& r, cDevElem& h, cdouble* pool, cdouble* s) {
+  entry_xform(r, h, pool);
+  apertures(r, h, pool);
+  if (BEND) edge(r, h.edge & 0xff, s[S_IRHO], s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
+  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
+}
+
+template <bool BEND>
+__device__ __forceinline__ void mpole_exit(R6& r, cDevElem& h, cdouble* pool, cdouble* s) {
+  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
+  if (BEND) edge(r, (h.edge >> 8) & 0xff, s[S_IRHO], s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
+  apertures(r, h, pool);
+  exit_xform(r, h, pool);
+  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+}
+
+// StrMPoleSymplectic4Pass.c:57-104 and BndMPoleSymplectic4Pass.c:75-122 for
+// the P particles of a lane; U slices per loop iteration.
+template <bool BEND, int N, int P, int U>
+__device__ __forceinline__ void mpole(R6* r, cDevElem& h, cdouble* pool) {
   cdouble* s = pool + h.sc;
   cdouble* a = pool + h.poly;
   cdouble* b = a + h.order + 1;
   const int n = h.order;
-  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
-  const double pn = 1.0 / (1.0 + r.dp);
-  const double nl1 = s[S_L1] * pn;
-  const double nl2 = s[S_L2] * pn;
-  entry_xform(r, h, pool);
-  apertures(r, h, pool);
+  double pn[P], nl1[P], nl2[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    if (h.flags & F_SCALE) change_pref(r[q], s[S_SCALE]);
+    pn[q] = 1.0 / (1.0 + r[q].dp);  // before T1/R1, as the reference hoists it
+    nl1[q] = s[S_L1] * pn[q];
+    nl2[q] = s[S_L2] * pn[q];
+    mpole_entry<BEND>(r[q], h, pool, s);
+  }
   const double irho = s[S_IRHO];
-  if (BEND) edge(r, h.edge & 0xff, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
-  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
   if (h.ops >= 0) {
-    r = slices_ops<BEND, false>(r, a, b, n, pool + h.ops, h.nslice, pn, irho, 0.0);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+      r[q] = slices_ops<BEND, false>(r[q], a, b, n, pool + h.ops, h.nslice, pn[q], irho, 0.0);
   } else {
     const double k1 = s[S_K1], k2 = s[S_K2];
     const double ki1 = k1 * irho, ki2 = k2 * irho;
     double blo, alo;
     horner_addends<N>(a, b, n, blo, alo);
 #if ATX_STRICT
-    const double h2 = 0.0;
-#else
-    const double h2 = 0.5 / (1.0 + r.dp);  // delta is constant through the slices
-#endif
-    // two slices per iteration: 0.5 % faster on the large ring, same arithmetic
-#pragma unroll 2
-    for (int m = 0; m < h.nslice; m++) {
-      fastdrift(r, nl1, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r, a, b, n, blo, alo, k1);
-      fastdrift(r, nl2, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2); else str_kick<N>(r, a, b, n, blo, alo, k2);
-      fastdrift(r, nl2, h2);
-      if (BEND) bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r, a, b, n, blo, alo, k1);
-      fastdrift(r, nl1, h2);
+    // U slices per iteration: a constant-count inner loop, fully unrolled
+    // (the slices may hold wave-wide votes, which a runtime-count unroll
+    // cannot duplicate), then the remaining slices one by one
+    auto slice = [&]() {
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift(r[q], nl1[q], 0.0);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r[q], a, b, n, blo, alo, k1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift(r[q], nl2[q], 0.0);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k2, irho, ki2); else str_kick<N>(r[q], a, b, n, blo, alo, k2);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift(r[q], nl2[q], 0.0);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r[q], a, b, n, blo, alo, k1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift(r[q], nl1[q], 0.0);
+    };
+    int m = 0;
+    for (; m + U <= h.nslice; m += U) {
+#pragma unroll
+      for (int u = 0; u < U; u++) slice();
     }
+    for (; m < h.nslice; m++) slice();
+#else
+    double nh1[P], nh2[P];  // delta is constant through the slices
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      const double h2 = 0.5 / (1.0 + r[q].dp);
+      nh1[q] = nl1[q] * h2;
+      nh2[q] = nl2[q] * h2;
+    }
+    // U slices per iteration: a constant-count inner loop, fully unrolled
+    // (the slices may hold wave-wide votes, which a runtime-count unroll
+    // cannot duplicate), then the remaining slices one by one
+    auto slice = [&]() {
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift_h(r[q], nl1[q], nh1[q]);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r[q], a, b, n, blo, alo, k1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift_h(r[q], nl2[q], nh2[q]);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k2, irho, ki2); else str_kick<N>(r[q], a, b, n, blo, alo, k2);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift_h(r[q], nl2[q], nh2[q]);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1); else str_kick<N>(r[q], a, b, n, blo, alo, k1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) fastdrift_h(r[q], nl1[q], nh1[q]);
+    };
+    int m = 0;
+    for (; m + U <= h.nslice; m += U) {
+#pragma unroll
+      for (int u = 0; u < U; u++) slice();
+    }
+    for (; m < h.nslice; m++) slice();
+#endif
   }
-  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
-  if (BEND) edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
-  apertures(r, h, pool);
-  exit_xform(r, h, pool);
-  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+#pragma unroll
+  for (int q = 0; q < P; q++) mpole_exit<BEND>(r[q], h, pool, s);
 }
 
-// StrMPoleSymplectic4RadPass.c:61-110 and BndMPoleSymplectic4RadPass.c:76-123.
-template <bool BEND, int N>
-__device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
+// StrMPoleSymplectic4RadPass.c:61-110 and BndMPoleSymplectic4RadPass.c:76-123
+// for the P particles of a lane; U slices per loop iteration.
+template <bool BEND, int N, int P, int U>
+__device__ __forceinline__ void mpole_rad(R6* r, cDevElem& h, cdouble* pool) {
   cdouble* s = pool + h.sc;
   cdouble* a = pool + h.poly;
   cdouble* b = a + h.order + 1;
   const int n = h.order;
-  if (h.flags & F_SCALE) change_pref(r, s[S_SCALE]);
-  entry_xform(r, h, pool);
-  apertures(r, h, pool);
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    if (h.flags & F_SCALE) change_pref(r[q], s[S_SCALE]);
+    mpole_entry<BEND>(r[q], h, pool, s);
+  }
   const double irho = s[S_IRHO];
-  if (BEND) edge(r, h.edge & 0xff, irho, s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
-  if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
   const double radc = s[S_RADC];
   if (h.ops >= 0) {
-    r = slices_ops<BEND, true>(r, a, b, n, pool + h.ops, h.nslice, 0.0, irho, radc);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+      r[q] = slices_ops<BEND, true>(r[q], a, b, n, pool + h.ops, h.nslice, 0.0, irho, radc);
   } else {
     const double l1 = s[S_L1], l2 = s[S_L2], k1 = s[S_K1], k2 = s[S_K2];
     const double ki1 = k1 * irho, ki2 = k2 * irho;
@@ -583,54 +769,95 @@ __device__ __forceinline__ void mpole_rad(R6& r, cDevElem& h, cdouble* pool) {
     horner_addends<N>(a, b, n, blo, alo);
 #if ATX_STRICT
     double pn = 0.0;  // unused: the strict steps divide as the reference does
-    // two slices per iteration: 0.5 % faster on the large ring, same arithmetic
-#pragma unroll 2
-    for (int m = 0; m < h.nslice; m++) {
-      rad_drift(r, l1);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, 0.0, pn);
-      rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k2, radc, 0.0, pn);
-      rad_drift(r, l2);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, 0.0, pn);
-      rad_drift(r, l1);
+    // U slices per iteration: a constant-count inner loop, fully unrolled
+    // (the slices may hold wave-wide votes, which a runtime-count unroll
+    // cannot duplicate), then the remaining slices one by one
+    auto slice = [&]() {
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift(r[q], l1);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
+        else rad_str_kick<N>(r[q], a, b, n, blo, alo, k1, radc, 0.0, pn);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift(r[q], l2);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick<N>(r[q], a, b, n, blo, alo, k2, irho, ki2, radc, 0.0, pn);
+        else rad_str_kick<N>(r[q], a, b, n, blo, alo, k2, radc, 0.0, pn);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift(r[q], l2);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick<N>(r[q], a, b, n, blo, alo, k1, irho, ki1, radc, 0.0, pn);
+        else rad_str_kick<N>(r[q], a, b, n, blo, alo, k1, radc, 0.0, pn);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift(r[q], l1);
+    };
+    int m = 0;
+    for (; m + U <= h.nslice; m += U) {
+#pragma unroll
+      for (int u = 0; u < U; u++) slice();
     }
+    for (; m < h.nslice; m++) slice();
 #else
-    const double rk1 = s[S_RK1], rk2 = s[S_RK2];
-    double pn = 1.0 / (1.0 + r.dp);  // one division per magnet
-    // two slices per iteration: 0.5 % faster on the large ring, same arithmetic
-#pragma unroll 2
-    for (int m = 0; m < h.nslice; m++) {
-      rad_drift_pn(r, l1, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, rk1, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, rk1, pn);
-      rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k2, irho, ki2, radc, rk2, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k2, radc, rk2, pn);
-      rad_drift_pn(r, l2, pn);
-      if (BEND) rad_bnd_kick<N>(r, a, b, n, blo, alo, k1, irho, ki1, radc, rk1, pn);
-      else rad_str_kick<N>(r, a, b, n, blo, alo, k1, radc, rk1, pn);
-      rad_drift_pn(r, l1, pn);
+    const double rk1 = s[S_RK1], rk2 = s[S_RK2], hl1 = s[S_HL1], hl2 = s[S_HL2];
+    Ang g[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) g[q] = to_angles(r[q]);  // one reciprocal per magnet
+    // U slices per iteration: a constant-count inner loop, fully unrolled
+    // (the slices may hold wave-wide votes, which a runtime-count unroll
+    // cannot duplicate), then the remaining slices one by one
+    auto slice = [&]() {
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift_ang(r[q], g[q], l1, hl1);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k1, irho, ki1, rk1);
+        else rad_str_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k1, rk1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift_ang(r[q], g[q], l2, hl2);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k2, irho, ki2, rk2);
+        else rad_str_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k2, rk2);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift_ang(r[q], g[q], l2, hl2);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (BEND) rad_bnd_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k1, irho, ki1, rk1);
+        else rad_str_kick_ang<N>(r[q], g[q], a, b, n, blo, alo, k1, rk1);
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++) rad_drift_ang(r[q], g[q], l1, hl1);
+    };
+    int m = 0;
+    for (; m + U <= h.nslice; m += U) {
+#pragma unroll
+      for (int u = 0; u < U; u++) slice();
     }
+    for (; m < h.nslice; m++) slice();
+#pragma unroll
+    for (int q = 0; q < P; q++) from_angles(r[q], g[q]);
 #endif
   }
-  if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
-  if (BEND) edge(r, (h.edge >> 8) & 0xff, irho, s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
-  apertures(r, h, pool);
-  exit_xform(r, h, pool);
-  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+#pragma unroll
+  for (int q = 0; q < P; q++) mpole_exit<BEND>(r[q], h, pool, s);
 }
 
-template <bool BEND, bool RAD>
-__device__ __forceinline__ void mpole_dispatch(R6& r, cDevElem& h, cdouble* pool) {
+template <bool BEND, bool RAD, int P, int U>
+__device__ __forceinline__ void mpole_dispatch(R6* r, cDevElem& h, cdouble* pool) {
   switch (h.order) {
-    case 0: RAD ? mpole_rad<BEND, 0>(r, h, pool) : mpole<BEND, 0>(r, h, pool); break;
-    case 1: RAD ? mpole_rad<BEND, 1>(r, h, pool) : mpole<BEND, 1>(r, h, pool); break;
-    case 2: RAD ? mpole_rad<BEND, 2>(r, h, pool) : mpole<BEND, 2>(r, h, pool); break;
-    case 3: RAD ? mpole_rad<BEND, 3>(r, h, pool) : mpole<BEND, 3>(r, h, pool); break;
-    default: RAD ? mpole_rad<BEND, -1>(r, h, pool) : mpole<BEND, -1>(r, h, pool); break;
+    case 0: RAD ? mpole_rad<BEND, 0, P, U>(r, h, pool) : mpole<BEND, 0, P, U>(r, h, pool); break;
+    case 1: RAD ? mpole_rad<BEND, 1, P, U>(r, h, pool) : mpole<BEND, 1, P, U>(r, h, pool); break;
+    case 2: RAD ? mpole_rad<BEND, 2, P, U>(r, h, pool) : mpole<BEND, 2, P, U>(r, h, pool); break;
+    case 3: RAD ? mpole_rad<BEND, 3, P, U>(r, h, pool) : mpole<BEND, 3, P, U>(r, h, pool); break;
+    default: RAD ? mpole_rad<BEND, -1, P, U>(r, h, pool) : mpole<BEND, -1, P, U>(r, h, pool); break;
   }
 }
 
@@ -844,7 +1071,7 @@ __device__ __forceinline__ void corrector(R6& r, cDevElem& h, cdouble* pool) {
   if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
 }
 
-#if ATX_STRICT  // diffusion matrices: built with the reference operation order only
+#if ATX_AUX  // diffusion matrices: built with the reference operation order only
 // --- Diffusion matrices -----------------------------------------------------
 // Restatement of the reference's Ohmi diffusion-matrix propagation
 // (diff_drift.c, diff_thinkick.c, diff_str_kick.c, diff_bnd_kick.c,
@@ -1149,22 +1376,14 @@ __global__ void __launch_bounds__(64) diffusion_kernel(DiffArgs args) {
   for (int i = 0; i < 36; i++) out[i] = b[i];
 }
 
-#endif  // ATX_STRICT
+#endif  // ATX_AUX
 
-// EXT: the kernel variant that also runs CavityPass and the exact passes.
-// Lattices without them use the variant without, whose element switch keeps
+// The kinds other than drifts and thick multipoles, for one particle. EXT:
+// the kernel variant that also runs CavityPass and the exact passes;
+// lattices without them use the variant without, whose element switch keeps
 // the register demand of the common passes only.
 template <bool EXT>
-__device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int nturn) {
-  if (h.kind == K_DRIFT) {  // DriftPass.c:16-48; half the elements of a ring, tested first
-    const uint64_t bits = ((uint64_t)(uint32_t)h.fringe << 32) | (uint32_t)h.edge;
-    entry_xform(r, h, pool);
-    apertures(r, h, pool);
-    drift6(r, __builtin_bit_cast(double, bits));
-    apertures(r, h, pool);
-    exit_xform(r, h, pool);
-    return;
-  }
+__device__ __forceinline__ void element_misc(R6& r, cDevElem& h, cdouble* pool, int nturn) {
   if (EXT) {
     switch (h.kind) {
       case K_CAVITY: r = cavity(r, pool + h.sc, (h.flags & F_THICK) != 0); return;
@@ -1187,15 +1406,44 @@ __device__ __forceinline__ void element(R6& r, cDevElem& h, cdouble* pool, int n
       apertures(r, h, pool);
       exit_xform(r, h, pool);
       break;
-    case K_STR: mpole_dispatch<false, false>(r, h, pool); break;
-    case K_BND: mpole_dispatch<true, false>(r, h, pool); break;
-    case K_STRRAD: mpole_dispatch<false, true>(r, h, pool); break;
-    case K_BNDRAD: mpole_dispatch<true, true>(r, h, pool); break;
     case K_THIN: thin(r, h, pool); break;
     case K_RF: rf_cavity(r, h, pool, nturn); break;
     case K_CORR: corrector(r, h, pool); break;
     case K_APERTURE: rect_ap(r, pool + h.rap); break;  // AperturePass.c:9-26
     default: break;
+  }
+}
+
+// The P particles of a lane through one element: drifts and thick
+// multipoles step all P together, the other kinds run per particle.
+template <bool EXT, int P, int U>
+__device__ __forceinline__ void element_lane(R6* r, cDevElem& h, cdouble* pool, int nturn) {
+  if (h.kind == K_DRIFT) {  // DriftPass.c:16-48; half the elements of a ring, tested first
+    const uint64_t bits = ((uint64_t)(uint32_t)h.fringe << 32) | (uint32_t)h.edge;
+    const double len = __builtin_bit_cast(double, bits);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      entry_xform(r[q], h, pool);
+      apertures(r[q], h, pool);
+    }
+#pragma unroll
+    for (int q = 0; q < P; q++) drift6(r[q], len);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      apertures(r[q], h, pool);
+      exit_xform(r[q], h, pool);
+    }
+    return;
+  }
+  switch (h.kind) {
+    case K_STR: mpole_dispatch<false, false, P, U>(r, h, pool); break;
+    case K_BND: mpole_dispatch<true, false, P, U>(r, h, pool); break;
+    case K_STRRAD: mpole_dispatch<false, true, P, U>(r, h, pool); break;
+    case K_BNDRAD: mpole_dispatch<true, true, P, U>(r, h, pool); break;
+    default:
+      element_misc<EXT>(r[0], h, pool, nturn);
+      if constexpr (P > 1) element_misc<EXT>(r[1], h, pool, nturn);
+      break;
   }
 }
 
@@ -1215,13 +1463,13 @@ __device__ __forceinline__ void store6(double* base, int64_t cs, const R6& r) {
   base[5 * cs] = r.ct;
 }
 
-// Dynamic LDS of a workgroup of B lanes, used to compact surviving particles
-// into the leading waves: 6*B doubles of state, B ints of particle index
-// (bit 31: loss already recorded), 2 x 8 per-wave counts (double-buffered
-// by round, so that a wave can publish its next count while a slower wave
-// still reads the current ones).
-__host__ __device__ constexpr size_t compact_lds_bytes(int block) {
-  return (size_t)block * (6 * sizeof(double) + sizeof(int)) + 2 * 8 * sizeof(int);
+// Dynamic LDS of a workgroup of G particle slots, used to compact surviving
+// particles into the leading lanes: 6*G doubles of state, G ints of particle
+// index (bit 31: loss already recorded), 2 x 8 per-wave counts
+// (double-buffered by round, so that a wave can publish its next count while
+// a slower wave still reads the current ones).
+__host__ __device__ constexpr size_t compact_lds_bytes(int slots) {
+  return (size_t)slots * (6 * sizeof(double) + sizeof(int)) + 2 * 8 * sizeof(int);
 }
 
 __device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
@@ -1235,26 +1483,30 @@ __device__ __forceinline__ void store_state(cInstance& I, int p, const R6& r) {
   rp[5 * cs] = r.ct;
 }
 
-// live_begin (repacked turn blocks after the first; NULL otherwise): first
-// workgroup of each instance among the workgroups that hold survivors, with
-// live_begin[ninst] their total and live_begin[ninst + 1] the particles per
-// workgroup (the block size, or 64 when few survive), so that the live
-// workgroups of all instances are the leading ones of the grid and the
-// hardware spreads them evenly over the CUs; the other workgroups end at
-// once.
-template <bool EXT>
+__device__ __forceinline__ R6 lost_state() { return R6{__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0}; }
+
+// P particles per lane, lane-major: lane l of a workgroup holds its slots
+// l*P .. l*P+P-1. live_begin (repacked turn blocks after the first; NULL
+// otherwise): first workgroup of each instance among the workgroups that
+// hold survivors, with live_begin[ninst] their total and live_begin[ninst +
+// 1] the particle slots per workgroup (blockDim.x * P, or 64 * P when few
+// survive), so that the live workgroups of all instances are the leading
+// ones of the grid and the hardware spreads them evenly over the CUs; the
+// other workgroups end at once.
+template <bool EXT, int P, int U>
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst,
                                                     const int32_t* __restrict__ live_begin) {
+  static_assert(P == 1 || P == 2, "one or two particles per lane");
   extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
   // Instance of this workgroup: the last whose first block is <= blockIdx.x.
   int lo = 0, hi = ninst - 1;
   const int bid = (int)blockIdx.x;
   int wbegin;
-  int B = (int)blockDim.x;  // particle slots per workgroup
+  int G = (int)blockDim.x * P;  // particle slots per workgroup
   if (live_begin) {
     if (bid >= live_begin[ninst]) return;
-    B = live_begin[ninst + 1];
+    G = live_begin[ninst + 1];
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (live_begin[mid] <= bid) lo = mid; else hi = mid - 1;
@@ -1273,69 +1525,87 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   // lanes of a repacked launch take the surviving particles idx[0..nlive)
   const int32_t* idx = I.idx;
   const int nslot = idx ? __builtin_amdgcn_readfirstlane(*I.nlive) : npart;
-  const int first = (bid - wbegin) * B;  // first slot of this workgroup
-  // waves of this workgroup that hold particles (uniform); the others end now
-  int nw = (min(B, nslot - first) + 63) >> 6;
+  const int first = (bid - wbegin) * G;  // first slot of this workgroup
+  // particles of this workgroup, and its waves that hold some (uniform);
+  // the other waves end now
+  int held = max(0, min(G, nslot - first));
+  int nw = (held + 64 * P - 1) / (64 * P);
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave >= nw) return;
 
-  int p = first + tid;
-  bool has = p < nslot;  // this lane holds a particle
-  if (has && idx) p = idx[p];
   cDevElem* elems = (cDevElem*)I.elems;
   cdouble* pool = (cdouble*)I.pool;
   cint* slots = (cint*)I.slots;
   const int64_t cs = I.cstride;
-  R6 r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (has) {
-    const double* rp = I.r + (int64_t)p * I.pstride;
-    r.x = rp[0];
-    r.px = rp[cs];
-    r.y = rp[2 * cs];
-    r.py = rp[3 * cs];
-    r.dp = rp[4 * cs];
-    r.ct = rp[5 * cs];
+  const int losses = I.losses;
+  int p[P];
+  bool has[P], lost[P];
+  R6 r[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    const int sl = tid * P + q;
+    has[q] = sl < held;  // this slot holds a particle
+    p[q] = has[q] ? (idx ? idx[first + sl] : first + sl) : 0;
+    r[q] = lost_state();
+    if (has[q]) {
+      const double* rp = I.r + (int64_t)p[q] * I.pstride;
+      r[q].x = rp[0];
+      r[q].px = rp[cs];
+      r[q].y = rp[2 * cs];
+      r[q].py = rp[3 * cs];
+      r[q].dp = rp[4 * cs];
+      r[q].ct = rp[5 * cs];
+    }
+    lost[q] = !has[q];  // slots without a particle never record a loss
+    if (has[q]) {
+      if (losses == 2) {
+        lost[q] = I.lost_flag[p[q]] != 0;  // recorded by an earlier call
+      } else if (losses) {
+        I.lost_flag[p[q]] = 0;
+        I.lost_turn[p[q]] = 0;
+        I.lost_elem[p[q]] = 0;
+        store6(I.lost_coord + 6 * (int64_t)p[q], 1, R6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+      }
+    }
   }
 
   const int nelem = I.nelem;
-  const int losses = I.losses;
   const bool check = !I.elempass;
   double* rout = I.rout;
   const int64_t np6 = 6 * (int64_t)npart;
-  bool lost = !has;  // lanes without a particle never record a loss
-  if (has) {
-    if (losses == 2) {
-      lost = I.lost_flag[p] != 0;  // recorded by an earlier call
-    } else if (losses) {
-      I.lost_flag[p] = 0;
-      I.lost_turn[p] = 0;
-      I.lost_elem[p] = 0;
-      R6 z = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      store6(I.lost_coord + 6 * (int64_t)p, 1, z);
-    }
-  }
   const int nturns = I.nturns;
   const bool refs = (I.n_first | I.n_std) != 0;  // uniform: skip the slot reads without refpts
   // Lost particles are dropped at turn boundaries and the survivors packed
-  // into the leading waves, so that waves left without particles end and
-  // free their SIMD slots. Not with refpts: lost particles still fill their
-  // refpt copies, as in at.c.
+  // into the leading lanes: with one particle per lane when packing frees a
+  // wave (the waves left without particles end and free their SIMD slots),
+  // with several whenever one was lost (a lane steps its particles together
+  // only while none of them is lost). Not with refpts: lost particles still
+  // fill their refpt copies, as in at.c.
   const bool compact = check && !refs && nturns > 1;
   double* s_r = lds;
-  int* s_p = (int*)(lds + 6 * B);
-  int* s_cnt = s_p + B;
+  int* s_p = (int*)(lds + 6 * G);
+  int* s_cnt = s_p + G;
   for (int t = 0; t < nturns; t++) {
     if (compact && t > 0) {
       // A particle whose x is NaN is lost for good (elements skip it and the
       // loss check leaves it as it is): store its final state now.
-      if (has && __builtin_isnan(r.x)) {
-        store_state(I, p, r);
-        has = false;
+      int mine = 0, below = 0;
+      int within[P];
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        if (has[q] && __builtin_isnan(r[q].x)) {
+          store_state(I, p[q], r[q]);
+          has[q] = false;
+        }
+        const uint64_t m = __ballot(has[q]);
+        within[q] = mine;
+        mine += has[q] ? 1 : 0;
+        below += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
       }
-      const uint64_t m = __ballot(has);
-      const int cnt = __popcll(m);
-      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+      int cnt = mine;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
       int* cnt_buf = s_cnt + 8 * (t & 1);
       if ((tid & 63) == 0) cnt_buf[wave] = cnt;
       __syncthreads();
@@ -1346,34 +1616,42 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
         total += c;
       }
       total = __builtin_amdgcn_readfirstlane(total);
-      const int need = (total + 63) >> 6;
-      if (need < nw) {  // uniform: packing frees at least one wave
-        if (has) {
-          const int j = off + rank;
-          s_r[j] = r.x;
-          s_r[B + j] = r.px;
-          s_r[2 * B + j] = r.y;
-          s_r[3 * B + j] = r.py;
-          s_r[4 * B + j] = r.dp;
-          s_r[5 * B + j] = r.ct;
-          s_p[j] = p | (lost ? (int)0x80000000u : 0);
+      const int need = (total + 64 * P - 1) / (64 * P);
+      if (need < nw || (P > 1 && total < held)) {  // uniform
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+          if (has[q]) {
+            const int j = off + below + within[q];
+            s_r[j] = r[q].x;
+            s_r[G + j] = r[q].px;
+            s_r[2 * G + j] = r[q].y;
+            s_r[3 * G + j] = r[q].py;
+            s_r[4 * G + j] = r[q].dp;
+            s_r[5 * G + j] = r[q].ct;
+            s_p[j] = p[q] | (lost[q] ? (int)0x80000000u : 0);
+          }
         }
         __syncthreads();
-        has = tid < total;
-        if (has) {
-          r.x = s_r[tid];
-          r.px = s_r[B + tid];
-          r.y = s_r[2 * B + tid];
-          r.py = s_r[3 * B + tid];
-          r.dp = s_r[4 * B + tid];
-          r.ct = s_r[5 * B + tid];
-          const int q = s_p[tid];
-          p = q & 0x7fffffff;
-          lost = q < 0;
-        } else {
-          r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
-          lost = true;
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+          const int j = tid * P + q;
+          has[q] = j < total;
+          if (has[q]) {
+            r[q].x = s_r[j];
+            r[q].px = s_r[G + j];
+            r[q].y = s_r[2 * G + j];
+            r[q].py = s_r[3 * G + j];
+            r[q].dp = s_r[4 * G + j];
+            r[q].ct = s_r[5 * G + j];
+            const int v = s_p[j];
+            p[q] = v & 0x7fffffff;
+            lost[q] = v < 0;
+          } else {
+            r[q] = lost_state();
+            lost[q] = true;
+          }
         }
+        held = total;
         nw = need;
         if (wave >= nw) return;  // this wave holds no particle any more
       }
@@ -1384,50 +1662,108 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
     const int64_t base = (t == 0) ? 0 : (int64_t)I.n_first + (int64_t)(t - 1) * I.n_std;
     const int e1 = I.end_elem;
     for (int e = e0; e < e1; e++) {
-      if (refs && has) {
+      if (refs) {
         const int sl = slot[e];
-        if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+#pragma unroll
+        for (int q = 0; q < P; q++)
+          if (has[q] && sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p[q], 1, r[q]);
       }
       cDevElem& h = elems[e];
-      if (!__builtin_isnan(r.x)) element<EXT>(r, h, pool, nturn);
-      if (check) {
-        if (losses) {
-          if (!lost && is_lost(r)) {
-            lost = true;
-            I.lost_flag[p] = 1;
-            I.lost_turn[p] = (uint32_t)nturn;
-            I.lost_elem[p] = (uint32_t)e;
-            store6(I.lost_coord + 6 * (int64_t)p, 1, r);
-            r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};
+      if (P == 1) {
+        if (!__builtin_isnan(r[0].x)) element_lane<EXT, 1, U>(r, h, pool, nturn);
+      } else {
+        // A lane steps its particles together. Every element skips a
+        // particle whose x is NaN (at.c:142-161, *Pass.c): in a wave that
+        // holds such a particle (a loss since the last compaction, or a NaN
+        // input), its state is kept in the lane's own LDS slots and put back
+        // after the element. Slots without a particle step too; they are
+        // never stored.
+        bool stale[P];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+          stale[q] = has[q] && __builtin_isnan(r[q].x);
+          any |= stale[q];
+        }
+        const bool keep = __ballot(any) != 0;  // uniform
+        if (keep) {
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            const int j = tid * P + q;
+            if (stale[q]) {
+              s_r[j] = r[q].x;
+              s_r[G + j] = r[q].px;
+              s_r[2 * G + j] = r[q].y;
+              s_r[3 * G + j] = r[q].py;
+              s_r[4 * G + j] = r[q].dp;
+              s_r[5 * G + j] = r[q].ct;
+            }
           }
-        } else if (__builtin_isfinite(r.x) && is_lost(r)) {
-          r = {__builtin_nan(""), 0.0, 0.0, 0.0, 0.0, 0.0};  // at.c:142-161
+        }
+        element_lane<EXT, P, U>(r, h, pool, nturn);
+        if (keep) {
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            const int j = tid * P + q;
+            if (stale[q]) {
+              r[q].x = s_r[j];
+              r[q].px = s_r[G + j];
+              r[q].y = s_r[2 * G + j];
+              r[q].py = s_r[3 * G + j];
+              r[q].dp = s_r[4 * G + j];
+              r[q].ct = s_r[5 * G + j];
+            }
+          }
+        }
+      }
+      if (check) {
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+          if (losses) {
+            if (!lost[q] && is_lost(r[q])) {
+              lost[q] = true;
+              I.lost_flag[p[q]] = 1;
+              I.lost_turn[p[q]] = (uint32_t)nturn;
+              I.lost_elem[p[q]] = (uint32_t)e;
+              store6(I.lost_coord + 6 * (int64_t)p[q], 1, r[q]);
+              r[q] = lost_state();
+            }
+          } else if (has[q] && __builtin_isfinite(r[q].x) && is_lost(r[q])) {
+            r[q] = lost_state();  // at.c:142-161
+          }
         }
       }
     }
-    if (refs && has) {
+    if (refs) {
       const int sl = slot[nelem];
-      if (sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p, 1, r);
+#pragma unroll
+      for (int q = 0; q < P; q++)
+        if (has[q] && sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p[q], 1, r[q]);
     }
   }
-  if (has) store_state(I, p, r);
+#pragma unroll
+  for (int q = 0; q < P; q++)
+    if (has[q]) store_state(I, p[q], r[q]);
 }
 
 }  // namespace ATX_NS
 }  // namespace atx
 
+// The kernel of this object's launch shape: nblocks workgroups of `block`
+// lanes, block * ATX_PPT particle slots each.
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
                                  int ext, const int32_t* live_begin, hipStream_t stream) {
+  const size_t lds = atx::ATX_NS::compact_lds_bytes(block * ATX_PPT);
   if (ext)
-    hipLaunchKernelGGL(atx::ATX_NS::track_kernel<true>, dim3(nblocks), dim3(block),
-                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst, live_begin);
+    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<true, ATX_PPT, ATX_UNROLL>), dim3(nblocks),
+                       dim3(block), lds, stream, d_insts, ninst, live_begin);
   else
-    hipLaunchKernelGGL(atx::ATX_NS::track_kernel<false>, dim3(nblocks), dim3(block),
-                       atx::ATX_NS::compact_lds_bytes(block), stream, d_insts, ninst, live_begin);
+    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<false, ATX_PPT, ATX_UNROLL>), dim3(nblocks),
+                       dim3(block), lds, stream, d_insts, ninst, live_begin);
   return hipGetLastError();
 }
 
-#if ATX_STRICT
+#if ATX_AUX
 namespace atx {
 // Repack between the turn blocks of a batched launch: one workgroup per
 // instance lists, in order, the particles still alive (x not NaN: a lost
@@ -1488,7 +1824,7 @@ __device__ __forceinline__ int live_count(const Instance& I) {
 // workgroups for either). Chunks of 1,024 instances: a wave-level scan, then
 // the wave totals through LDS, carrying the running sum from chunk to chunk.
 __global__ void __launch_bounds__(1024) live_map_kernel(const Instance* __restrict__ insts, int ninst,
-                                                        int block, int fine_limit,
+                                                        int block, int fine, int fine_limit,
                                                         int32_t* __restrict__ live_begin) {
   __shared__ int s_wave[16];
   __shared__ int s_total;
@@ -1509,7 +1845,7 @@ __global__ void __launch_bounds__(1024) live_map_kernel(const Instance* __restri
     s_total = t;
   }
   __syncthreads();
-  const int g = s_total <= fine_limit ? 64 : block;
+  const int g = s_total <= fine_limit ? fine : block;
   int carry = 0;
   for (int base = 0; base < ninst; base += (int)blockDim.x) {
     const int i = base + tid;
@@ -1548,10 +1884,11 @@ extern "C" hipError_t atx_launch_repack(const atx::Instance* d_insts, int ninst,
 }
 
 extern "C" hipError_t atx_launch_live_map(const atx::Instance* d_insts, int ninst, int block,
-                                          int fine_limit, int32_t* live_begin, hipStream_t stream) {
+                                          int fine, int fine_limit, int32_t* live_begin,
+                                          hipStream_t stream) {
   if (ninst <= 0) return hipSuccess;
   hipLaunchKernelGGL(atx::live_map_kernel, dim3(1), dim3(1024), 0, stream, d_insts, ninst, block,
-                     fine_limit, live_begin);
+                     fine, fine_limit, live_begin);
   return hipGetLastError();
 }
 

@@ -40,7 +40,11 @@ EXPORTS = (
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
     "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
-    "atx_diffusion_matrices", "atx_adjoint_batch", "atx_shutdown",
+    "atx_diffusion_matrices", "atx_adjoint_batch", "atx_shutdown", "atx_get_device",
+    "atx_device_alloc", "atx_device_free", "atx_host_alloc", "atx_host_free", "atx_memcpy",
+    "atx_stream_create", "atx_stream_destroy", "atx_stream_synchronize", "atx_stream_wait_event",
+    "atx_event_create", "atx_event_destroy", "atx_event_record", "atx_event_query",
+    "atx_event_synchronize", "atx_event_elapsed", "atx_lattice_set_launch_shape",
 )
 
 
@@ -51,6 +55,20 @@ class BatchItem(C.Structure):
                 ("pad", C.c_uint32), ("lost_flag", C.c_void_p), ("lost_turn", C.c_void_p),
                 ("lost_elem", C.c_void_p), ("lost_coord", C.c_void_p), ("nturns", C.c_int32),
                 ("turn", C.c_int32)]
+
+class LaunchShape(C.Structure):
+    """struct atx_launch_shape of include/atx.h (zero fields: engine default)."""
+
+    _fields_ = [("block", C.c_int32), ("ppt", C.c_int32), ("unroll", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+def launch_shape(block: int = 0, ppt: int = 0, unroll: int = 0):
+    """A LaunchShape, or None (the engine's default) when every field is 0."""
+    if not (block or ppt or unroll):
+        return None
+    return LaunchShape(int(block), int(ppt), int(unroll), 0)
+
 
 class AdjointItem(C.Structure):
     """struct atx_adjoint_item of include/atx.h."""
@@ -67,20 +85,33 @@ _lib = None
 _lib_lock = threading.Lock()
 
 
-def _bind_torch_runtime() -> None:
-    """Keep one HIP runtime in the process.
+def _bind_runtime() -> None:
+    """Keep one HIP runtime in the process, without importing torch.
 
     PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7, which
-    torch requests as "libamdhip64.so"). If torch is loaded first,
-    libatx.so's request for libamdhip64.so.7 resolves to that same runtime,
-    so torch's device pointers and streams are valid in the engine. If the
-    engine were loaded first, torch would load a second runtime beside it,
-    and that one fails to initialise ("No HIP GPUs are available"). Torch is
-    optional: without it the engine uses the system runtime alone."""
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    torch loads from its lib directory as "libamdhip64.so"). A process in
+    which both the engine and torch run must hold one runtime: torch's
+    device pointers and streams are then valid in the engine, and torch,
+    imported after the engine, does not start a second runtime (which fails
+    to initialise: "No HIP GPUs are available"). So, when torch is
+    installed, that library file is loaded (globally, by path, before
+    libatx.so resolves its libamdhip64.so.7) -- the runtime only, not torch.
+    ``ATX_HIP_RUNTIME=system`` skips this and uses the system ROCm runtime
+    (for processes that never use torch); without torch installed that is
+    the runtime anyway."""
+    import importlib.util
+    import sys
+
+    if "torch" in sys.modules or os.environ.get("ATX_HIP_RUNTIME", "") == "system":
+        return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for base in spec.submodule_search_locations:
+        lib = os.path.join(base, "lib", "libamdhip64.so")
+        if os.path.exists(lib):
+            C.CDLL(lib, mode=C.RTLD_GLOBAL)
+            return
 
 
 def load_library(path: str | None = None) -> C.CDLL:
@@ -93,7 +124,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         p = path or LIB_PATH
         if not os.path.exists(p):
             raise ImportError(f"at_amd engine library not built: {p} (run python -m at_amd.build)")
-        _bind_torch_runtime()
+        _bind_runtime()
         lib = C.CDLL(p)
         lib.atx_last_error.restype = C.c_char_p
         lib.atx_lattice_create.argtypes = [C.POINTER(C.c_void_p)]
@@ -111,8 +142,9 @@ def load_library(path: str | None = None) -> C.CDLL:
         lib.atx_elempass.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_double,
                                      C.c_double, C.c_void_p, C.c_uint32]
         lib.atx_track_batch.argtypes = [C.POINTER(BatchItem), C.c_int32, C.c_int32, C.c_int32,
-                                        C.c_double, C.c_double, C.c_int32, C.c_int32,
-                                        C.c_void_p, C.c_uint32]
+                                        C.c_double, C.c_double, C.c_int32,
+                                        C.POINTER(LaunchShape), C.c_void_p, C.c_uint32]
+        lib.atx_lattice_set_launch_shape.argtypes = [C.c_void_p, C.POINTER(LaunchShape)]
         lib.atx_lattice_turn_counter.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.atx_lattice_set_integrator.argtypes = [C.c_void_p, C.c_int32]
         lib.atx_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint32,
@@ -132,6 +164,23 @@ def load_library(path: str | None = None) -> C.CDLL:
                                         C.POINTER(C.c_int), C.c_char_p, C.c_int]
         lib.atx_set_device.argtypes = [C.c_int]
         lib.atx_shutdown.argtypes = []
+        vp, pvp = C.c_void_p, C.POINTER(C.c_void_p)
+        lib.atx_get_device.argtypes = [C.POINTER(C.c_int)]
+        lib.atx_device_alloc.argtypes = [pvp, C.c_uint64]
+        lib.atx_device_free.argtypes = [vp]
+        lib.atx_host_alloc.argtypes = [pvp, C.c_uint64]
+        lib.atx_host_free.argtypes = [vp]
+        lib.atx_memcpy.argtypes = [vp, vp, C.c_uint64, vp]
+        lib.atx_stream_create.argtypes = [pvp]
+        lib.atx_stream_destroy.argtypes = [vp]
+        lib.atx_stream_synchronize.argtypes = [vp]
+        lib.atx_stream_wait_event.argtypes = [vp, vp]
+        lib.atx_event_create.argtypes = [pvp]
+        lib.atx_event_destroy.argtypes = [vp]
+        lib.atx_event_record.argtypes = [vp, vp]
+        lib.atx_event_query.argtypes = [vp, C.POINTER(C.c_int)]
+        lib.atx_event_synchronize.argtypes = [vp]
+        lib.atx_event_elapsed.argtypes = [C.POINTER(C.c_float), vp, vp]
         for name in EXPORTS:
             getattr(lib, name).restype = C.c_char_p if name == "atx_last_error" else C.c_int
         _lib = lib
@@ -216,6 +265,13 @@ class DeviceLattice:
             self.close()
         except Exception:
             pass
+
+    def set_launch_shape(self, block: int = 0, ppt: int = 0, unroll: int = 0) -> None:
+        """Launch shape of this lattice's atx_track/atx_elempass calls (all 0:
+        the engine's default)."""
+        sh = launch_shape(block, ppt, unroll)
+        _check(self.lib.atx_lattice_set_launch_shape(self.h, None if sh is None else C.byref(sh)),
+               self.lib)
 
     def set_integrator(self, integrator: int) -> None:
         """Integrator type 1-6 of atgpu's SymplecticIntegrator (4 = CPU)."""
@@ -344,15 +400,16 @@ class DeviceLattice:
 
 def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
                 rest_energy: float = 0.0, losses: bool = True, block_size: int = 0,
-                stream: int | None = None, strict: bool = False, soa: bool = False,
-                continue_losses: bool = False, repack: bool = False) -> None:
+                ppt: int = 0, unroll: int = 0, stream: int | None = None, strict: bool = False,
+                soa: bool = False, continue_losses: bool = False, repack: bool = False) -> None:
     """One launch over many (DeviceLattice, device particle buffer) pairs.
 
     items: sequence of dicts with keys ``lattice`` (DeviceLattice), ``r``
     (device pointer), ``n`` (particles), with losses ``lost`` = (flag,
     turn, elem, coord) device pointers, and optionally ``nturns``/``turn``
     for an item that tracks its own number of turns. ``repack``: turn blocks
-    with the survivors packed between them, run as one graph (ATX_REPACK)."""
+    with the survivors packed between them, run as one graph (ATX_REPACK).
+    block_size, ppt, unroll: the launch shape (0: the engine's default)."""
     lib = load_library()
     arr = (BatchItem * max(len(items), 1))()
     for i, it in enumerate(items):
@@ -366,9 +423,10 @@ def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
             b.lost_flag, b.lost_turn, b.lost_elem, b.lost_coord = it["lost"]
     flags = ((ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
              | (ATX_CONTINUE_LOSSES if continue_losses else 0) | (ATX_REPACK if repack else 0))
+    sh = launch_shape(block_size, ppt, unroll)
     _check(lib.atx_track_batch(arr, len(items), int(nturns), int(turn), float(energy),
-                               float(rest_energy), int(bool(losses)), int(block_size), stream,
-                               flags), lib)
+                               float(rest_energy), int(bool(losses)),
+                               None if sh is None else C.byref(sh), stream, flags), lib)
 
 
 def device_count() -> int:

@@ -28,12 +28,13 @@ per lattice (at.c:340-681).
 from __future__ import annotations
 
 import collections
+import ctypes as C
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
-from . import engine
+from . import device, engine
 
 BYTES_PER_PARTICLE = 48 + 1 + 4 + 4 + 48  # state + loss map (flag, turn, elem, coord)
 
@@ -69,24 +70,83 @@ class Result:
         return self.done - self.arrival
 
 
+class _Pool:
+    """Device and pinned host buffers kept for reuse, by power-of-two size
+    class: freeing device memory waits for the device, and pinning host
+    pages is slow, so a serving loop that allocated and freed per request
+    would stall its own queued launches."""
+
+    def __init__(self):
+        self.free = collections.defaultdict(list)
+
+    def take(self, nbytes: int):
+        size = 1 << max(12, (max(nbytes, 1) - 1).bit_length())
+        if self.free[size]:
+            return size, self.free[size].pop()
+        return size, (device.HostArray(size, np.uint8), device.DeviceArray(size, np.uint8))
+
+    def give(self, size: int, bufs) -> None:
+        self.free[size].append(bufs)
+
+
+class _Block:
+    """One request's device state in one allocation, mirrored by one pinned
+    host buffer: particles (N, 6) f64, loss coordinates (N, 6) f64, loss
+    turn and element (N,) u32, loss flag (N,) u8 -- 105 bytes per particle,
+    every part at an offset aligned to its type. One copy in (particles and
+    a zeroed loss map), one copy out."""
+
+    def __init__(self, n: int, pool: _Pool):
+        self.n, self.pool = n, pool
+        self.nbytes = BYTES_PER_PARTICLE * n
+        self.off = {"r": 0, "coord": 48 * n, "turn": 96 * n, "elem": 100 * n, "flag": 104 * n}
+        self.size, (self.host, self.dev) = pool.take(self.nbytes)
+
+    def upload(self, stream) -> None:
+        device.check(device.lib().atx_memcpy(self.dev.ptr, self.host.ptr, self.nbytes,
+                                             stream.handle))
+
+    def download(self, stream) -> None:
+        device.check(device.lib().atx_memcpy(self.host.ptr, self.dev.ptr, self.nbytes,
+                                             stream.handle))
+
+    def ptr(self, part: str) -> int:
+        return self.dev.ptr + self.off[part]
+
+    def view(self, part: str) -> np.ndarray:
+        n, h, o = self.n, self.host.numpy, self.off[part]
+        if part in ("r", "coord"):
+            return h[o:o + 48 * n].view(np.float64).reshape(n, 6)
+        if part in ("turn", "elem"):
+            return h[o:o + 4 * n].view(np.uint32)
+        return h[o:o + n]
+
+    def stage(self, particles: np.ndarray) -> None:
+        """Host side of the copy in: the particles and a zero loss map."""
+        self.host.numpy[48 * self.n:self.nbytes] = 0
+        self.view("r")[:] = np.asarray(particles).T
+
+    def result(self) -> tuple:
+        """(particles (6, N), loss map) from the host side after the copy out."""
+        return (self.view("r").T.copy(),
+                {"islost": self.view("flag").astype(bool), "turn": self.view("turn").copy(),
+                 "elem": self.view("elem").copy(), "coord": self.view("coord").T.copy()})
+
+    def close(self) -> None:
+        """Back to the pool; only once the copies and launches using it
+        have completed."""
+        if self.pool is not None:
+            self.pool.give(self.size, (self.host, self.dev))
+            self.pool = None
+
+
 @dataclass
 class _Slot:
     req: Request
-    r: object                 # torch (N, 6) float64
-    flag: object
-    turn: object
-    elem: object
-    coord: object
+    mem: _Block
     done_turns: int = 0
     admitted: float = 0.0
     status: str = "done"
-    host: dict = field(default_factory=dict)
-
-
-def _pinned(shape, dtype):
-    import torch
-
-    return torch.empty(shape, dtype=dtype, pin_memory=True)
 
 
 class Server:
@@ -104,8 +164,6 @@ class Server:
                  hbm_budget: int = 64 << 30, max_active: int = 256, turn_block: int = 128,
                  block_size: int = 0, depth: int = 2, strict: bool = False,
                  repack: bool = True):
-        import torch
-
         self.dls = [l if isinstance(l, engine.DeviceLattice) else engine.DeviceLattice(l)
                     for l in lattices]
         self.energy = float(energy)
@@ -117,9 +175,12 @@ class Server:
         self.depth = max(1, int(depth))
         self.strict = bool(strict)
         self.repack = bool(repack)
-        self.device = torch.device("cuda", torch.cuda.current_device())
-        self.stream = torch.cuda.Stream(device=self.device)      # launches
-        self.copy_stream = torch.cuda.Stream(device=self.device)  # device-to-host results
+        dev = C.c_int(0)
+        engine._check(engine.load_library().atx_get_device(C.byref(dev)), engine.load_library())
+        self.device = int(dev.value)
+        self.stream = device.Stream()       # copies in and launches
+        self.copy_stream = device.Stream()  # device-to-host results
+        self.pool = _Pool()
         for dl in self.dls:  # compile and upload every instance now, not on first use
             dl.check(self.energy, self.rest)
         self.launches = 0
@@ -129,53 +190,31 @@ class Server:
         return req.npart * BYTES_PER_PARTICLE
 
     def _admit(self, req: Request, now: float) -> _Slot:
-        import torch
-
-        n = req.npart
-        dev = self.device
-        host = _pinned((n, 6), torch.float64)
-        host.numpy()[:] = np.asarray(req.particles).T
-        with torch.cuda.stream(self.stream):
-            r = torch.empty((n, 6), dtype=torch.float64, device=dev)
-            r.copy_(host, non_blocking=True)
-            s = _Slot(req, r, torch.zeros(n, dtype=torch.uint8, device=dev),
-                      torch.zeros(n, dtype=torch.int32, device=dev),
-                      torch.zeros(n, dtype=torch.int32, device=dev),
-                      torch.zeros((n, 6), dtype=torch.float64, device=dev), admitted=now)
-        s.host["in"] = host  # keep the pinned source alive until the copy ran
-        return s
+        mem = _Block(req.npart, self.pool)
+        mem.stage(req.particles)
+        mem.upload(self.stream)
+        return _Slot(req, mem, admitted=now)
 
     def _launch(self, active):
         items = []
         for s in active:
             nt = min(self.turn_block, s.req.nturns - s.done_turns)
-            items.append({"lattice": self.dls[s.req.instance], "r": s.r.data_ptr(),
+            m = s.mem
+            items.append({"lattice": self.dls[s.req.instance], "r": m.ptr("r"),
                           "n": s.req.npart, "nturns": nt, "turn": s.done_turns,
-                          "lost": (s.flag.data_ptr(), s.turn.data_ptr(), s.elem.data_ptr(),
-                                   s.coord.data_ptr())})
+                          "lost": (m.ptr("flag"), m.ptr("turn"), m.ptr("elem"),
+                                   m.ptr("coord"))})
             s.done_turns += nt
         engine.track_batch(items, 0, energy=self.energy, rest_energy=self.rest, losses=True,
-                           block_size=self.block_size, stream=self.stream.cuda_stream,
+                           block_size=self.block_size, stream=self.stream.handle,
                            strict=self.strict, continue_losses=True, repack=self.repack)
         self.launches += 1
 
     def _copy_back(self, s: _Slot):
-        import torch
-
-        ev = torch.cuda.Event()
-        ev.record(self.stream)
-        self.copy_stream.wait_event(ev)
-        with torch.cuda.stream(self.copy_stream):
-            out = {k: _pinned(t.shape, t.dtype) for k, t in
-                   (("r", s.r), ("flag", s.flag), ("turn", s.turn), ("elem", s.elem),
-                    ("coord", s.coord))}
-            for k, t in (("r", s.r), ("flag", s.flag), ("turn", s.turn), ("elem", s.elem),
-                         ("coord", s.coord)):
-                out[k].copy_(t, non_blocking=True)
-                t.record_stream(self.copy_stream)
-            done = torch.cuda.Event(enable_timing=True)
-            done.record(self.copy_stream)
-        s.host.update(out)
+        ev = device.Event().record(self.stream)
+        self.copy_stream.wait(ev)
+        s.mem.download(self.copy_stream)
+        done = device.Event().record(self.copy_stream)
         return ev, done
 
     def run(self, requests, realtime: bool = True, log=None):
@@ -223,9 +262,7 @@ class Server:
         device = self.device
 
         def work():
-            import torch
-
-            torch.cuda.set_device(device)
+            engine.set_device(device)
             try:
                 self._serve(self._fetch_online, self._deliver_online)
             except BaseException as exc:  # noqa: BLE001 - reported through the futures
@@ -317,17 +354,16 @@ class Server:
         arrival), more arrivals possible); block=True when the server has
         nothing to do until a request arrives. deliver(result) receives each
         completed request. Returns the wall time."""
-        import torch
-
         queue = collections.deque()
         active: list[_Slot] = []
         finishing = collections.deque()   # (slot, finish event, copy-done event)
         inflight = collections.deque()    # launch-end events
         arrival = {}
         used = 0
-        torch.cuda.synchronize()
-        t0_ev = torch.cuda.Event(enable_timing=True)
-        t0_ev.record(self.stream)
+        device.synchronize()
+        self.stream.synchronize()
+        self.copy_stream.synchronize()
+        t0_ev = device.Event().record(self.stream)
         t0 = time.perf_counter()
         self._loop_t0_offset = t0 - getattr(self, "_t0", t0)
 
@@ -364,6 +400,8 @@ class Server:
                         finishing.append((sl, fin, cp))
                     elif why:
                         # admitted but not launched yet: its buffers are free to go
+                        self.stream.synchronize()
+                        sl.mem.close()
                         used -= self.need(sl.req)
                         self._cancel.discard(sl.req.rid)
                         deliver(Result(sl.req.rid, np.asarray(sl.req.particles).copy(),
@@ -384,9 +422,7 @@ class Server:
             if active and len(inflight) < self.depth:
                 self._launch(active)
                 launched = True
-                ev = torch.cuda.Event()
-                ev.record(self.stream)
-                inflight.append(ev)
+                inflight.append(device.Event().record(self.stream))
                 still = []
                 for s in active:
                     if s.done_turns >= s.req.nturns:
@@ -398,16 +434,12 @@ class Server:
             # retire finished requests whose results are on the host
             while finishing and finishing[0][2].query():
                 s, _, cp = finishing.popleft()
-                done = t0_ev.elapsed_time(cp) / 1000.0  # results on the host
-                h = s.host
+                done = t0_ev.elapsed_ms(cp) / 1000.0  # results on the host
                 used -= self.need(s.req)
-                deliver(Result(
-                    s.req.rid, h["r"].numpy().T.copy(),
-                    {"islost": h["flag"].numpy().astype(bool),
-                     "turn": h["turn"].numpy().view(np.uint32).copy(),
-                     "elem": h["elem"].numpy().view(np.uint32).copy(),
-                     "coord": h["coord"].numpy().T.copy()},
-                    arrival.pop(s.req.rid), s.admitted, done, s.status, s.done_turns))
+                parts, lost = s.mem.result()
+                s.mem.close()
+                deliver(Result(s.req.rid, parts, lost, arrival.pop(s.req.rid), s.admitted, done,
+                               s.status, s.done_turns))
                 self._cancel.discard(s.req.rid)
             while inflight and inflight[0].query():
                 inflight.popleft()
@@ -423,7 +455,8 @@ class Server:
                 for q, t in new:
                     arrival[q.rid] = t
                     queue.append(q)
-        torch.cuda.synchronize()
+        self.stream.synchronize()
+        self.copy_stream.synchronize()
         return now()
 
 
@@ -431,18 +464,15 @@ def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realti
                      strict: bool = False, log=None):
     """Baseline: requests in arrival order, one atx_track launch each over
     all of its turns, results copied back before the next one starts."""
-    import torch
-
     dls = [l if isinstance(l, engine.DeviceLattice) else engine.DeviceLattice(l) for l in dls]
     for dl in dls:
         dl.check(energy, rest_energy)
-    dev = torch.device("cuda", torch.cuda.current_device())
-    stream = torch.cuda.Stream(device=dev)
+    stream = device.Stream()
+    pool = _Pool()
     order = sorted(requests, key=lambda q: (q.arrival, q.rid))
     results = {}
-    torch.cuda.synchronize()
-    t0_ev = torch.cuda.Event(enable_timing=True)
-    t0_ev.record(stream)
+    device.synchronize()
+    t0_ev = device.Event().record(stream)
     t0 = time.perf_counter()
     for q in order:
         if realtime:
@@ -450,36 +480,25 @@ def serve_one_by_one(dls, requests, *, energy: float, rest_energy: float, realti
             if wait > 0:
                 time.sleep(wait)
         admitted = time.perf_counter() - t0
-        n = q.npart
-        with torch.cuda.stream(stream):
-            host = _pinned((n, 6), torch.float64)
-            host.numpy()[:] = np.asarray(q.particles).T
-            r = host.to(dev, non_blocking=True)
-            lf = torch.empty(n, dtype=torch.uint8, device=dev)
-            lt = torch.empty(n, dtype=torch.int32, device=dev)
-            le = torch.empty(n, dtype=torch.int32, device=dev)
-            lc = torch.empty((n, 6), dtype=torch.float64, device=dev)
-            dls[q.instance].track_device(r.data_ptr(), n, q.nturns, energy=energy,
-                                         rest_energy=rest_energy, losses=True,
-                                         loss_ptrs=(lf.data_ptr(), lt.data_ptr(), le.data_ptr(),
-                                                    lc.data_ptr()),
-                                         stream=stream.cuda_stream, strict=strict)
-            out = [_pinned(t.shape, t.dtype) for t in (r, lf, lt, le, lc)]
-            for o, t in zip(out, (r, lf, lt, le, lc)):
-                o.copy_(t, non_blocking=True)
-            fin = torch.cuda.Event(enable_timing=True)
-            fin.record(stream)  # results on the host
+        mem = _Block(q.npart, pool)
+        mem.stage(q.particles)
+        mem.upload(stream)
+        dls[q.instance].track_device(mem.ptr("r"), q.npart, q.nturns, energy=energy,
+                                     rest_energy=rest_energy, losses=True,
+                                     loss_ptrs=(mem.ptr("flag"), mem.ptr("turn"),
+                                                mem.ptr("elem"), mem.ptr("coord")),
+                                     stream=stream.handle, strict=strict)
+        mem.download(stream)
+        fin = device.Event().record(stream)  # results on the host
         stream.synchronize()
-        done = t0_ev.elapsed_time(fin) / 1000.0
-        results[q.rid] = Result(q.rid, out[0].numpy().T.copy(),
-                                {"islost": out[1].numpy().astype(bool),
-                                 "turn": out[2].numpy().view(np.uint32).copy(),
-                                 "elem": out[3].numpy().view(np.uint32).copy(),
-                                 "coord": out[4].numpy().T.copy()},
-                                q.arrival if realtime else 0.0, admitted, done)
+        done = t0_ev.elapsed_ms(fin) / 1000.0
+        parts, lost = mem.result()
+        mem.close()
+        results[q.rid] = Result(q.rid, parts, lost, q.arrival if realtime else 0.0, admitted,
+                                done)
         if log is not None:
             log(results[q.rid])
-    torch.cuda.synchronize()
+    stream.synchronize()
     wall = time.perf_counter() - t0
     res = [results[q.rid] for q in requests]
     return res, _stats(res, wall, energy, len(order))

@@ -16,7 +16,14 @@ on the launch stream. cpu_baseline runs the reference integrator cores
 bounded sample of the same workload on the host cores.
 
     python bench.py [--steps 10] [--warmup 1] [--turns-per-step 100]
-    python bench.py --mode batch|serve|adjoint   (the other workloads, one line each)
+    python bench.py --mode batch|serve|adjoint|monitors   (the other workloads, one line each)
+
+The default line also carries ``serving``: one seeded trace of 48 requests
+(error realisations resident on the device, N log-uniform on [2e3, 2e5],
+turn counts scaled to {4, 16, 64} so that the offered loads 5, 15 and 50
+requests/s bracket the capacity of both modes), served by continuous
+batching and by one launch per request: p50/p99, throughput, the ratio at
+equal p99, and whether every result is identical.
 """
 
 from __future__ import annotations
@@ -36,26 +43,47 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "track_kernel_large_ring_traffic.j
 
 def measured_traffic(particles: int, turns: int):
     """HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs of the
-    same launch shape (copy-calibrated), or None when they were taken on
-    another shape."""
+    same launch shape (copy-calibrated), with the file they come from, or
+    (None, None) when they were taken on another shape."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
     if t.get("particles") != particles or t.get("turns_per_launch") != turns:
-        return None
-    return t.get("bytes_per_launch")
+        return None, None
+    return t.get("bytes_per_launch"), os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
 METRIC = "particle-element passes/s FP64 + %FP64 peak; served p50/p99 latency vs plain"
 
 
-def cpu_baseline(ring, beam_np, nturns: int, threads: int):
+def host_cores() -> dict:
+    """The host as the CPU baseline sees it: nproc, the cores this process
+    may use, the lease's thread budget (OMP_NUM_THREADS) and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(affinity, int(omp)) if omp.isdigit() and int(omp) > 0 else affinity
+    return {"nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": omp or None,
+            "threads": threads, "cpu_model": model}
+
+
+def cpu_baseline(ring, beam_np, nturns: int):
     import numpy as np
 
     from oracle import pyoracle
 
+    host = host_cores()
+    threads = host["threads"]
     impl = "ref" if pyoracle.available("ref") else "oracle"
     r = np.asfortranarray(beam_np.copy())
     t0 = time.perf_counter()
@@ -70,94 +98,99 @@ def cpu_baseline(ring, beam_np, nturns: int, threads: int):
         "cores": threads,
         "kind": "reference" if impl == "ref" else "port",
         "sample": f"{r.shape[1]} particles x {nturns} turns of the same ring and beam "
-                  f"({passes:.3e} passes in {dt:.1f} s, OpenMP {threads} threads)",
+                  f"({passes:.3e} passes in {dt:.1f} s, OpenMP {threads} threads: every core "
+                  f"the lease gives this process)",
+        "host": host,
     }
 
 
-def _events(n):
-    import torch
+def _pairs(n):
+    from at_amd import device
 
-    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-            for _ in range(n)]
+    return [(device.Event(), device.Event()) for _ in range(n)]
+
+
+def _same(a, b) -> bool:
+    import numpy as np
+
+    return bool(np.array_equal(a.to_host(), b.to_host(), equal_nan=True))
 
 
 def bench_batch(args) -> dict:
     """Batched error realisations: ``--instances`` error-seeded copies of the
     large ring, each with a DA polar grid of U[1,000, 40,000] particles, all
     tracked for ``--batch-turns`` turns (1,024 by default) in ONE launch
-    (workgroup size from the autotuner), against the same launch without
+    (launch shape from the autotuner), against the same launch without
     repacking and, with ``--per-instance``, the instances tracked one launch
-    each."""
+    each. ``--shift`` sets the rms quadrupole/sextupole shift (3 um residual
+    by default; 30 um is the uncorrected size)."""
     import numpy as np
-    import torch
 
-    from at_amd import engine
+    from at_amd import device, engine
     from at_amd.autotune import tune_batch
     from at_amd.batch import track_batch
     from at_amd.flops import tracked_flops_and_passes
     from at_amd.workloads import batched_da_workload
 
-    dev = torch.device("cuda:0")
-    work = batched_da_workload(ninst=args.instances, seed=7)
+    work = batched_da_workload(ninst=args.instances, seed=7, shift=args.shift)
     lats = [l for l, _ in work]
     dls = [engine.DeviceLattice(l) for l in lats]
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
-    base = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for _, p in work]
+    base = [device.DeviceArray.from_host(p.T) for _, p in work]
     for dl in dls:
         dl.check(energy, rest)
-    tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False)
-    block = tune["block_size"]
+    tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False,
+                      repack=not args.no_repack)
+    shape = {k: tune[k] for k in ("block_size", "ppt", "unroll") if k in tune}
     T = args.batch_turns
-    print(f"batch: autotuned workgroup {block}; {len(dls)} instances, {T} turns",
+    print(f"batch: autotuned {shape}; {len(dls)} instances, {T} turns",
           file=sys.stderr, flush=True)
-    stream = torch.cuda.current_stream()
-    parts = [b.clone() for b in base]
-    (e0, e1), = _events(1)
-    torch.cuda.synchronize()
+    stream = device.Stream()
+    parts = [b.clone(stream) for b in base]
+    (e0, e1), = _pairs(1)
+    stream.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
     lms = track_batch(dls, parts, T, energy=energy, rest_energy=rest, losses=True,
-                      block_size=block, repack=not args.no_repack)
+                      repack=not args.no_repack, stream=stream, **shape)
     e1.record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
+    ms = e0.elapsed_ms(e1)
     print(f"batch: launch {ms:.1f} ms", file=sys.stderr, flush=True)
     # the same launch without repacking between turn blocks
     plain_ms = plain_same = None
     if not args.no_repack and not args.no_compare:
-        parts3 = [b.clone() for b in base]
-        (p0, p1), = _events(1)
-        torch.cuda.synchronize()
+        parts3 = [b.clone(stream) for b in base]
+        (p0, p1), = _pairs(1)
+        stream.synchronize()
         p0.record(stream)
-        track_batch(dls, parts3, T, energy=energy, rest_energy=rest, losses=True, block_size=block)
+        track_batch(dls, parts3, T, energy=energy, rest_energy=rest, losses=True,
+                    stream=stream, **shape)
         p1.record(stream)
-        torch.cuda.synchronize()
-        plain_ms = p0.elapsed_time(p1)
+        stream.synchronize()
+        plain_ms = p0.elapsed_ms(p1)
         print(f"batch: plain launch {plain_ms:.1f} ms", file=sys.stderr, flush=True)
-        plain_same = all(torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0))
-                         for a, b in zip(parts, parts3))
+        plain_same = all(_same(a, b) for a, b in zip(parts, parts3))
     flops = passes = 0.0
     for lat, lm in zip(lats, lms):
-        fl, pa = tracked_flops_and_passes(lat, T, 0, lm.flag.cpu().numpy().astype(bool),
-                                          lm.turn.cpu().numpy().view(np.uint32),
-                                          lm.elem.cpu().numpy().view(np.uint32))
+        fl, pa = tracked_flops_and_passes(lat, T, 0, lm.flag.to_host().astype(bool),
+                                          lm.turn.to_host(), lm.elem.to_host())
         flops += fl
         passes += pa
-    seq_ms = None
+    seq_ms = same = None
     if not args.no_compare and args.per_instance:
-        parts2 = [b.clone() for b in base]
-        (s0, s1), = _events(1)
-        torch.cuda.synchronize()
+        parts2 = [b.clone(stream) for b in base]
+        (s0, s1), = _pairs(1)
+        stream.synchronize()
         s0.record(stream)
         for dl, p in zip(dls, parts2):
-            dl.track_device(p.data_ptr(), p.shape[0], T, energy=energy, rest_energy=rest,
-                            stream=stream.cuda_stream)
+            dl.track_device(p.ptr, p.shape[0], T, energy=energy, rest_energy=rest,
+                            stream=stream.handle)
         s1.record(stream)
-        torch.cuda.synchronize()
-        seq_ms = s0.elapsed_time(s1)
-        same = all(torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0))
-                   for a, b in zip(parts, parts2))
+        stream.synchronize()
+        seq_ms = s0.elapsed_ms(s1)
+        same = all(_same(a, b) for a, b in zip(parts, parts2))
     achieved = flops / (ms * 1e-3) / 1e12
     ntot = sum(int(b.shape[0]) for b in base)
     return {
@@ -168,88 +201,146 @@ def bench_batch(args) -> dict:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(dls)} error realisations of large_ring ({len(lats[0])} "
-                               f"elements; rolls 100 urad, gradients 1e-3, residual shifts "
-                               f"3 um), DA polar grids of U[1000, 40000] particles "
-                               f"({ntot} total), {T} turns, one launch"
+                               f"elements; rolls 100 urad, gradients 1e-3, shifts "
+                               f"{args.shift * 1e6:g} um rms), DA polar grids of U[1000, 40000] "
+                               f"particles ({ntot} total), {T} turns, one launch"
                                + ("" if args.no_repack else
                                   " (turn blocks with survivors repacked, one graph)"),
-                   "block_size": block, "autotune_ms": tune["ms"], "wall_s": wall},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                   "launch_shape": shape, "autotune_ms": tune["ms"], "wall_s": wall},
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
         "plain_launch_ms": plain_ms,
         "identical_to_plain_launch": plain_same,
         "one_launch_per_instance_ms": seq_ms,
         "speedup_vs_one_launch_per_instance": (seq_ms / ms) if seq_ms else None,
-        "identical_to_one_launch_per_instance": same if seq_ms else None,
+        "identical_to_one_launch_per_instance": same,
+    }
+
+
+def _serving_setup(n_instances: int):
+    from at_amd import engine
+    from at_amd.workloads import RESIDUAL_SHIFT, error_realisation, large_ring
+
+    base = large_ring()
+    lats = [error_realisation(base, 1000 + i, shift=RESIDUAL_SHIFT) for i in range(n_instances)]
+    dls = [engine.DeviceLattice(l) for l in lats]
+    for dl in dls:
+        dl.check(base.energy, base.particle.rest_energy)
+    return base, dls
+
+
+def _serve_both(base, dls, trace, args, log=None) -> dict:
+    """One trace through continuous batching and through one launch per
+    request: throughput, p50/p99 and whether every result is identical."""
+    import numpy as np
+
+    from at_amd.serving import Server, alive_passes, serve_one_by_one
+
+    energy, rest = base.energy, base.particle.rest_energy
+    nel = len(base)
+    srv = Server(dls, energy=energy, rest_energy=rest, turn_block=args.turn_block,
+                 repack=not args.no_repack)
+    res_b, st_b = srv.run(trace, log=log)
+    res_p, st_p = serve_one_by_one(dls, trace, energy=energy, rest_energy=rest, log=log)
+    passes = alive_passes(trace, res_b, lambda i: nel)
+    same = all(np.array_equal(a.particles, b.particles, equal_nan=True)
+               and np.array_equal(a.lost["islost"], b.lost["islost"])
+               and np.array_equal(a.lost["turn"], b.lost["turn"])
+               and np.array_equal(a.lost["elem"], b.lost["elem"])
+               for a, b in zip(res_b, res_p))
+
+    def side(st):
+        return {"passes_per_s": passes / st["wall_s"], "requests_per_s": len(trace) / st["wall_s"],
+                "p50_s": st["p50_s"], "p99_s": st["p99_s"], "wall_s": st["wall_s"],
+                "launches": st["launches"]}
+
+    return {"batched": side(st_b), "one_launch_per_request": side(st_p),
+            "identical": bool(same), "passes": passes}
+
+
+def serving_sweep(args) -> dict:
+    """The served half of the metric: one seeded request trace (same requests
+    at every rate, arrival times scaled) offered at rates that bracket the
+    capacity of both modes, served by continuous batching and by one plain
+    launch per request. ``at_equal_p99``: taking as latency bound the worst
+    p99 one-launch-per-request reached, the highest throughput each mode
+    delivered within it, and their ratio."""
+    from at_amd.workloads import serving_trace
+
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    turns = tuple(int(t) for t in args.serve_turns.split(","))
+    base, dls = _serving_setup(args.serve_instances)
+    # warm-up: a small request through each path
+    warm = serving_trace(n_requests=2, rate=1e6, seed=99, n_instances=len(dls), nmin=64,
+                         nmax=64, turns=(2,))
+    _serve_both(base, dls, warm, args)
+    points = []
+    for rate in rates:
+        trace = serving_trace(n_requests=args.serve_requests, rate=rate, seed=args.serve_seed,
+                              n_instances=len(dls), turns=turns)
+        print(f"serving: {len(trace)} requests at {rate:g}/s", file=sys.stderr, flush=True)
+        pt = _serve_both(base, dls, trace, args)
+        pt["offered_per_s"] = rate
+        points.append(pt)
+        b, p = pt["batched"], pt["one_launch_per_request"]
+        print(f"serving: {rate:g}/s batched {b['requests_per_s']:.2f} req/s p99 {b['p99_s']:.3f} s;"
+              f" plain {p['requests_per_s']:.2f} req/s p99 {p['p99_s']:.3f} s", file=sys.stderr,
+              flush=True)
+    bound = max(pt["one_launch_per_request"]["p99_s"] for pt in points)
+    best = {m: max([pt[m]["passes_per_s"] for pt in points if pt[m]["p99_s"] <= bound],
+                   default=None) for m in ("batched", "one_launch_per_request")}
+    ratio = (best["batched"] / best["one_launch_per_request"]
+             if best["batched"] and best["one_launch_per_request"] else None)
+    return {
+        "trace": {"requests": args.serve_requests, "seed": args.serve_seed,
+                  "instances": len(dls), "particles": "log-uniform [2e3, 2e5]",
+                  "turns": list(turns), "kinds": "half DA grids, half Gaussian beams",
+                  "lattices": f"error realisations of large_ring ({len(base)} elements), "
+                              f"resident on the device", "turn_block": args.turn_block},
+        "points": points,
+        "at_equal_p99": {"p99_bound_s": bound,
+                         "batched_passes_per_s": best["batched"],
+                         "one_launch_per_request_passes_per_s": best["one_launch_per_request"],
+                         "ratio": ratio},
+        "identical": all(pt["identical"] for pt in points),
     }
 
 
 def bench_serve(args) -> dict:
-    """Served mode: a Poisson trace of ``--requests`` tracking and DA requests
-    over ``--instances`` error realisations of the large ring, N log-uniform
-    on [2e3, 2e5], turns in {256, 1024, 4096}, served by continuous batching
-    and by one launch per request."""
-    import numpy as np
-    import torch
+    """Served mode at full request size: a Poisson trace of ``--requests``
+    tracking and DA requests over ``--instances`` error realisations of the
+    large ring, N log-uniform on [2e3, 2e5], turns in {256, 1024, 4096},
+    served by continuous batching and by one launch per request."""
+    from at_amd.workloads import serving_trace
 
-    from at_amd import engine
-    from at_amd.serving import Server, alive_passes, serve_one_by_one
-    from at_amd.workloads import RESIDUAL_SHIFT, error_realisation, large_ring, serving_trace
-
-    base = large_ring()
-    trace = serving_trace(n_requests=args.requests, rate=args.rate, n_instances=args.instances)
-    used = sorted({q.instance for q in trace})
-    lats = {i: error_realisation(base, 1000 + i, shift=RESIDUAL_SHIFT) for i in used}
-    dls = {i: engine.DeviceLattice(lats[i]) for i in used}
-    # remap instance indices onto the compiled subset
-    idx = {i: k for k, i in enumerate(used)}
-    for q in trace:
-        q.instance = idx[q.instance]
-    dl_list = [dls[i] for i in used]
-    energy, rest = base.energy, base.particle.rest_energy
-    srv = Server(dl_list, energy=energy, rest_energy=rest, turn_block=args.turn_block,
-                 repack=not args.no_repack)
-    # warm-up: one small request through each path
-    from at_amd.serving import Request
-    warm = [Request(0, 0, trace[0].particles[:, :64], 2)]
-    srv.run(warm, realtime=False)
-    serve_one_by_one(dl_list, warm, energy=energy, rest_energy=rest, realtime=False)
-    torch.cuda.synchronize()
+    base, dls = _serving_setup(args.instances)
+    trace = serving_trace(n_requests=args.requests, rate=args.rate, n_instances=len(dls))
+    warm = serving_trace(n_requests=2, rate=1e6, seed=99, n_instances=len(dls), nmin=64,
+                         nmax=64, turns=(2,))
+    _serve_both(base, dls, warm, args)
 
     def log(r):
         print(f"request {r.rid} done at {r.done:.2f} s (latency {r.latency:.2f} s)",
               file=sys.stderr, flush=True)
 
-    res_b, st_b = srv.run(trace, log=log)
-    nel = len(base)
-    passes = alive_passes(trace, res_b, lambda i: nel)
-    out = {
+    pt = _serve_both(base, dls, trace, args, log=log)
+    b, p = pt["batched"], pt["one_launch_per_request"]
+    return {
         "metric": "served tracking: particle-element passes/s, p50/p99 latency",
-        "value": passes / st_b["wall_s"],
+        "value": b["passes_per_s"],
         "unit": "particle-element passes/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": st_b["wall_s"] * 1e3,
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": b["wall_s"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(trace)} requests, Poisson {args.rate}/s, "
-                               f"{len(used)} error realisations of large_ring ({nel} elements; "
-                               f"rolls 100 urad, gradients 1e-3, residual shifts 3 um), "
-                               f"N log-uniform [2e3, 2e5], turns in {{256, 1024, 4096}}",
+                               f"{len(dls)} error realisations of large_ring ({len(base)} "
+                               f"elements; rolls 100 urad, gradients 1e-3, residual shifts "
+                               f"3 um), N log-uniform [2e3, 2e5], turns in {{256, 1024, 4096}}",
                    "turn_block": args.turn_block,
-                   "particles": int(sum(q.npart for q in trace)),
                    "requests": [[q.npart, q.nturns, q.kind] for q in trace]},
-        "batched": st_b,
+        "batched": b, "one_launch_per_request": p,
+        "throughput_ratio": p["wall_s"] / b["wall_s"], "identical_results": pt["identical"],
     }
-    if not args.no_compare:
-        res_1, st_1 = serve_one_by_one(dl_list, trace, energy=energy, rest_energy=rest, log=log)
-        same = all(np.array_equal(a.particles, b.particles, equal_nan=True)
-                   and np.array_equal(a.lost["turn"], b.lost["turn"])
-                   and np.array_equal(a.lost["elem"], b.lost["elem"])
-                   for a, b in zip(res_b, res_1))
-        out["one_launch_per_request"] = st_1
-        out["one_launch_per_request"]["value"] = passes / st_1["wall_s"]
-        out["throughput_ratio"] = st_1["wall_s"] / st_b["wall_s"]
-        out["identical_results"] = bool(same)
-    return out
 
 
 def bench_monitors(args) -> dict:
@@ -259,13 +350,12 @@ def bench_monitors(args) -> dict:
     ``--steps`` launches of one turn each. The copies are 48 B per particle
     and refpt; the line reports their HBM bandwidth against 8 TB/s."""
     import numpy as np
-    import torch
 
-    from at_amd import engine
+    from at_amd import device, engine
+    from at_amd.batch import LossMap
     from at_amd.flops import tracked_flops_and_passes
     from at_amd.workloads import gaussian_beam, large_ring
 
-    dev = torch.device("cuda:0")
     ring = large_ring()
     nelem = len(ring)
     n = args.particles
@@ -273,41 +363,35 @@ def bench_monitors(args) -> dict:
     R = refs.size
     beam = gaussian_beam(n, seed=0, scale=10.0)
     dl = engine.DeviceLattice(ring)
-    stream = torch.cuda.current_stream()
-    r_dev = torch.from_numpy(np.ascontiguousarray(beam.T)).to(dev)
-    rout = torch.empty((R, n, 6), dtype=torch.float64, device=dev)
+    stream = device.Stream()
+    r_dev = device.DeviceArray.from_host(beam.T)
+    rout = device.DeviceArray((R, n, 6), np.float64)
     K, W = args.steps, args.warmup
-    lf = torch.zeros(n, dtype=torch.uint8, device=dev)
-    lt = torch.zeros(n, dtype=torch.int32, device=dev)
-    le = torch.zeros(n, dtype=torch.int32, device=dev)
-    lc = torch.zeros((n, 6), dtype=torch.float64, device=dev)
+    lm = LossMap.empty(n)
 
     def launch(step):
-        dl.track_device(r_dev.data_ptr(), n, 1, refpts=refs, rout_ptr=rout.data_ptr(),
-                        turn=step, energy=ring.energy, rest_energy=ring.particle.rest_energy,
-                        losses=True, loss_ptrs=(lf.data_ptr(), lt.data_ptr(), le.data_ptr(),
-                                                lc.data_ptr()),
-                        stream=stream.cuda_stream, strict=args.strict)
+        dl.track_device(r_dev.ptr, n, 1, refpts=refs, rout_ptr=rout.ptr, turn=step,
+                        energy=ring.energy, rest_energy=ring.particle.rest_energy, losses=True,
+                        loss_ptrs=lm.ptrs(), stream=stream.handle, strict=args.strict)
 
     for s in range(W):
         launch(s)
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(K)]
+    stream.synchronize()
+    ev = _pairs(K)
     flops = []
-    alive_before = ~lf.cpu().numpy().astype(bool)
+    alive_before = ~lm.flag.to_host().astype(bool)
     for k in range(K):
         ev[k][0].record(stream)
         launch(W + k)
         ev[k][1].record(stream)
-        torch.cuda.synchronize()
-        lost = lf.cpu().numpy().astype(bool)
+        stream.synchronize()
+        lost = lm.flag.to_host().astype(bool)
         fl, _ = tracked_flops_and_passes(ring, 1, W + k, (lost & alive_before)[alive_before],
-                                         lt.cpu().numpy().view(np.uint32)[alive_before],
-                                         le.cpu().numpy().view(np.uint32)[alive_before])
+                                         lm.turn.to_host()[alive_before],
+                                         lm.elem.to_host()[alive_before])
         flops.append(fl)
         alive_before = ~lost
-    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
     # per launch: state in and out, the refpt copies, the loss map zeroed
     # (57 B per particle) and written for lost particles
     nbytes = n * (48 + 48 + 57) + 48 * n * R
@@ -340,6 +424,10 @@ def bench_adjoint(args) -> dict:
     from at_amd import lattice as L
     from at_amd.adjoint import Problem, TrackingLoss, sweep_plan
     from at_amd.workloads import gaussian_beam, large_ring
+
+    def _events(k):
+        return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                for _ in range(k)]
 
     ring = large_ring(radiation=False, cavity=True)
     params = []
@@ -414,7 +502,7 @@ def bench_adjoint(args) -> dict:
         from oracle import pyoracle
 
         impl = "ref" if pyoracle.available("ref") else "oracle"
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = host_cores()["threads"]
         m = min(n, args.cpu_particles)
         r = np.asfortranarray(beam[:, :m].copy())
         t0 = time.perf_counter()
@@ -434,17 +522,29 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint", "monitors"),
                     default="track",
-                    help="track (default): the large-ring tracking line; batch: many error "
-                         "realisations in one launch; serve: continuous batching of a request "
-                         "trace; adjoint: strength gradients; monitors: refpts at every "
-                         "element (HBM bandwidth of the copies)")
+                    help="track (default): the large-ring tracking line with the serving sweep "
+                         "and the CPU baseline; batch: many error realisations in one launch; "
+                         "serve: continuous batching of a full-size request trace; adjoint: "
+                         "strength gradients; monitors: refpts at every element (HBM "
+                         "bandwidth of the copies)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-turns", type=int, default=1024)
+    ap.add_argument("--shift", type=float, default=3e-6,
+                    help="batch: rms quadrupole/sextupole shift of the error realisations [m]")
     ap.add_argument("--no-repack", action="store_true",
                     help="batch/serve: one plain launch instead of repacked turn blocks")
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--turn-block", type=int, default=128)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="track: leave out the serving sweep")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rates", default="5,15,50",
+                    help="track: offered loads of the serving sweep [requests/s]")
+    ap.add_argument("--serve-turns", default="4,16,64",
+                    help="track: turn counts of the serving sweep's requests")
+    ap.add_argument("--serve-instances", type=int, default=64)
+    ap.add_argument("--serve-seed", type=int, default=11)
     ap.add_argument("--adj-particles", type=int, default=4096)
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--no-compare", action="store_true")
@@ -471,9 +571,10 @@ def main() -> int:
         return 2
 
     import numpy as np
-    import torch
 
     if args.copy_probe:
+        import torch
+
         src = torch.ones(args.copy_probe * 131072, dtype=torch.float64, device="cuda:0")
         dst = torch.empty_like(src)
         for _ in range(3):
@@ -482,12 +583,11 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes": src.numel() * 8, "dispatches": 3}))
         return 0
 
-    from at_amd import engine
+    from at_amd import device, engine
+    from at_amd.batch import LossMap
     from at_amd.flops import lattice_flops, tracked_flops_and_passes
     from at_amd.workloads import gaussian_beam, large_ring
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     engine.set_device(0)
     if args.mode != "track":
         fn = {"batch": bench_batch, "serve": bench_serve, "adjoint": bench_adjoint,
@@ -500,54 +600,47 @@ def main() -> int:
     T = args.turns_per_step
     beam = gaussian_beam(n, seed=0, scale=10.0)
     dl = engine.DeviceLattice(ring)
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
+    stream = device.Stream()
 
     # particles resident in HBM: (N, 6) row-major == (6, N) F-order
-    r_dev = torch.from_numpy(np.ascontiguousarray(beam.T)).to(dev)
+    r_dev = device.DeviceArray.from_host(beam.T)
     K, W = args.steps, args.warmup
-    loss_sets = []
-    for _ in range(W + K):
-        loss_sets.append((torch.zeros(n, dtype=torch.uint8, device=dev),
-                          torch.zeros(n, dtype=torch.int32, device=dev),
-                          torch.zeros(n, dtype=torch.int32, device=dev),
-                          torch.zeros((n, 6), dtype=torch.float64, device=dev)))
+    loss_sets = [LossMap.empty(n) for _ in range(W + K)]
 
     def launch(step: int):
-        lf, lt, le, lc = loss_sets[step]
-        dl.track_device(r_dev.data_ptr(), n, T, turn=step * T, energy=ring.energy,
+        dl.track_device(r_dev.ptr, n, T, turn=step * T, energy=ring.energy,
                         rest_energy=ring.particle.rest_energy, losses=True,
-                        loss_ptrs=(lf.data_ptr(), lt.data_ptr(), le.data_ptr(), lc.data_ptr()),
-                        stream=sptr, strict=args.strict)
+                        loss_ptrs=loss_sets[step].ptrs(), stream=stream.handle,
+                        strict=args.strict)
 
+    print(f"track: {W} warm-up and {K} timed launches of {T} turns, {n} particles",
+          file=sys.stderr, flush=True)
     for s in range(W):
         launch(s)
-    torch.cuda.synchronize()
+    stream.synchronize()
 
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(K)]
-    torch.cuda.synchronize()
+    events = _pairs(K)
     t0 = time.perf_counter()
     for k in range(K):
         events[k][0].record(stream)
         launch(W + k)
         events[k][1].record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = [a.elapsed_time(b) for a, b in events]
+    kernel_ms = [a.elapsed_ms(b) for a, b in events]
 
     # algorithmic work of each timed launch, from its loss map
     lost_before = np.zeros(n, bool)
     for s in range(W):
-        lost_before |= loss_sets[s][0].cpu().numpy().astype(bool)
+        lost_before |= loss_sets[s].flag.to_host().astype(bool)
     flops_tot = passes_tot = 0.0
     flops_per_launch = []
     for k in range(K):
-        lf, lt, le, _ = (x.cpu().numpy() for x in loss_sets[W + k])
+        lm = loss_sets[W + k]
+        lf, lt, le = lm.flag.to_host(), lm.turn.to_host(), lm.elem.to_host()
         lost = lf.astype(bool) & ~lost_before
         alive = ~lost_before
-        fl, pa = tracked_flops_and_passes(ring, T, (W + k) * T, lost[alive],
-                                          lt.view(np.uint32)[alive], le.view(np.uint32)[alive])
+        fl, pa = tracked_flops_and_passes(ring, T, (W + k) * T, lost[alive], lt[alive], le[alive])
         flops_per_launch.append(fl)
         flops_tot += fl
         passes_tot += pa
@@ -555,6 +648,9 @@ def main() -> int:
     survivors = int((~lost_before).sum())
     mean_ms = float(np.mean(kernel_ms))
     achieved = float(np.mean(flops_per_launch)) / (mean_ms * 1e-3) / 1e12
+    traffic, traffic_src = (None, None) if args.strict else measured_traffic(n, T)
+    print(f"track: {mean_ms:.1f} ms per launch, {achieved:.2f} TFLOP/s", file=sys.stderr,
+          flush=True)
     line = {
         "metric": METRIC,
         "value": passes_tot / wall,
@@ -579,21 +675,28 @@ def main() -> int:
             "flop_per_particle_turn": float(lattice_flops(ring).sum()),
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "fp64 vector (VALU); MI355X FP64 vector peak = FP64 matrix peak",
+            "bound": "fp64_valu",
+            "pipe": "FP64 vector ALU (no MFMA: the path is not a contraction); MI355X FP64 "
+                    "vector peak = FP64 matrix peak",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": measured_traffic(n, T) if not args.strict else None,
+            "traffic": traffic,
+            "traffic_source": (f"{traffic_src}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
+                               f"this launch shape, copy-calibrated; not measured in this run"
+                               if traffic_src else None),
             "kernel_ms_mean": mean_ms,
             "flops_per_launch": float(np.mean(flops_per_launch)),
         },
     }
+    if not args.no_serve:
+        line["serving"] = serving_sweep(args)
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
         sample = beam[:, : args.cpu_particles]
-        line["cpu_baseline"] = cpu_baseline(ring, sample, args.cpu_turns, threads)
+        print(f"cpu baseline: {sample.shape[1]} particles x {args.cpu_turns} turns",
+              file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(ring, sample, args.cpu_turns)
     print(json.dumps(line), flush=True)
     return 0
 

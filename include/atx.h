@@ -114,6 +114,25 @@ int atx_track(atx_lattice *lat, double *r_in, uint32_t num_particles, int32_t nt
 int atx_elempass(atx_lattice *lat, int32_t index, double *r_in, uint32_t num_particles,
                  double energy, double rest_energy, void *stream, uint32_t flags);
 
+/* Launch shape of the tracking kernel (no counterpart in the reference,
+ * whose CUDA launch is fixed at 64 threads per block, one particle per
+ * thread: atgpu/CudaGPU.cpp:157-185). block: lanes per workgroup, a
+ * multiple of 64 in [64, 512]; ppt: particles per lane, 1 or 2 (a lane
+ * steps its particles through each magnet together); unroll: slices per
+ * loop iteration of the thick multipoles, 1, 2 or 4 (1 or 2 with ppt 2). A
+ * zero field takes the engine's default. Results do not depend on the
+ * shape. */
+typedef struct atx_launch_shape {
+    int32_t block;
+    int32_t ppt;
+    int32_t unroll;
+    int32_t pad;
+} atx_launch_shape;
+
+/* Shape of the atx_track / atx_elempass launches on this lattice (NULL:
+ * back to the engine's default). */
+int atx_lattice_set_launch_shape(atx_lattice *lat, const atx_launch_shape *shape);
+
 /* --- Batched instances ---------------------------------------------------
  * Many lattice instances (e.g. error realisations), each with its own
  * device-resident particles, tracked in ONE launch: workgroups are assigned
@@ -124,7 +143,7 @@ int atx_elempass(atx_lattice *lat, int32_t index, double *r_in, uint32_t num_par
  * which tracks one lattice per call (at.c:340, PyInterface.cpp:132); each
  * instance has the semantics of atx_track with refpts = [] and start/end =
  * whole ring. All pointers are device pointers; the call is asynchronous on
- * stream. block_size: workgroup size (64..512, multiple of 64; 0 = 256). */
+ * stream. shape: launch shape (NULL: the engine's default). */
 typedef struct atx_batch_item {
     atx_lattice *lattice;
     double *r;              /* (6,N) F-order, or SoA with ATX_SOA */
@@ -139,8 +158,8 @@ typedef struct atx_batch_item {
 } atx_batch_item;
 
 int atx_track_batch(const atx_batch_item *items, int32_t num_items, int32_t nturns, int32_t turn,
-                    double energy, double rest_energy, int32_t losses, int32_t block_size,
-                    void *stream, uint32_t flags);
+                    double energy, double rest_energy, int32_t losses,
+                    const atx_launch_shape *shape, void *stream, uint32_t flags);
 
 /* --- Differentiation -----------------------------------------------------
  * Reverse-mode gradient through nturns turns of the whole ring. No
@@ -231,6 +250,36 @@ int atx_device_count(int *count);
 int atx_device_info(int device, char *name, int name_len, char *arch, int arch_len,
                     int *compute_units, char *platform, int platform_len);
 int atx_set_device(int device);
+int atx_get_device(int *device);
+
+/* --- Device memory, pinned host memory, streams and events ----------------
+ * Replaces the memory half of atgpu's GPU context (allocDevice, freeDevice,
+ * hostToDevice, deviceToHost: atgpu/AbstractGPU.h:31-40, used by
+ * Lattice::fillGPUMemory, atgpu/Lattice.cpp:272-303), and adds the stream
+ * and event handles that asynchronous callers (the batching and serving
+ * loops) need to keep state resident on the device and overlap copies with
+ * tracking without a second runtime. Handles are opaque: `stream` and
+ * `event` are a hipStream_t and a hipEvent_t; a NULL stream is the null
+ * stream. Streams are created non-blocking, at the default priority.
+ * atx_memcpy copies in either direction (the pointers say which) and is
+ * asynchronous on `stream` (synchronous with respect to the host only for
+ * pageable host memory). atx_event_query sets *done to 1 when the work
+ * before the event has completed, 0 otherwise. */
+int atx_device_alloc(void **ptr, uint64_t bytes);
+int atx_device_free(void *ptr);
+int atx_host_alloc(void **ptr, uint64_t bytes);
+int atx_host_free(void *ptr);
+int atx_memcpy(void *dst, const void *src, uint64_t bytes, void *stream);
+int atx_stream_create(void **stream);
+int atx_stream_destroy(void *stream);
+int atx_stream_synchronize(void *stream);
+int atx_stream_wait_event(void *stream, void *event);
+int atx_event_create(void **event);
+int atx_event_destroy(void *event);
+int atx_event_record(void *event, void *stream);
+int atx_event_query(void *event, int *done);
+int atx_event_synchronize(void *event);
+int atx_event_elapsed(float *ms, void *start, void *end);
 
 #ifdef __cplusplus
 }

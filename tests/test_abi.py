@@ -131,7 +131,7 @@ def test_batch_argument_errors():
     item.lattice = None
     item.r = None
     item.num_particles = 10
-    rc = lib.atx_track_batch(C.byref(item), 1, 5, 0, 3e9, 0.0, 0, 0, None, 0)
+    rc = lib.atx_track_batch(C.byref(item), 1, 5, 0, 3e9, 0.0, 0, None, None, 0)
     assert rc == engine.ATX_E_VALUE
 
 
@@ -181,3 +181,15 @@ def test_parallel_adjoint_sweep_cost_model():
 
 def test_device_count_without_gpu_is_zero_or_more():
     assert engine.device_count() >= 0
+
+
+def test_invalid_launch_shapes_are_value_errors():
+    """atx_lattice_set_launch_shape validates on the host: workgroup size a
+    multiple of 64 in [64, 512], 1 or 2 particles per lane, 1, 2 or 4 slices
+    per iteration (1 or 2 with two particles per lane)."""
+    dl = engine.DeviceLattice([Drift("d", 1.0)])
+    for shape in ((96, 1, 1), (1024, 1, 1), (256, 3, 1), (256, 1, 3), (256, 2, 4)):
+        with pytest.raises(ValueError):
+            dl.set_launch_shape(*shape)
+    for shape in ((64, 1, 1), (512, 2, 2), (0, 2, 0), (0, 0, 0)):
+        dl.set_launch_shape(*shape)

@@ -7,7 +7,8 @@ build, what the oracle gives (bit-exact, RF cavity included)."""
 import numpy as np
 import pytest
 
-from at_amd import engine
+from at_amd import device, engine
+from at_amd.device import DeviceArray
 from at_amd.autotune import tune_batch
 from at_amd.batch import LossMap, track_batch
 from at_amd.workloads import da_polar_grid, error_realisation, large_ring
@@ -26,25 +27,23 @@ def _instances(n_inst=3, ncells=2, cavity=True):
 
 @pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
 def test_batch_equals_single_and_oracle(strict):
-    import torch
 
     lats, parts = _instances()
     dls = [engine.DeviceLattice(l) for l in lats]
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
-    dev = torch.device("cuda:0")
-    tens = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    tens = [DeviceArray.from_host(p.T) for p in parts]
     lms = track_batch(dls, tens, 7, turn=2, energy=energy, rest_energy=rest, strict=strict,
                       block_size=128)
-    torch.cuda.synchronize()
+    device.synchronize()
     for i, (lat, p) in enumerate(zip(lats, parts)):
-        got = tens[i].cpu().numpy().T
+        got = tens[i].to_host().T
         single = p.copy(order="F")
         _, lm1 = dls[i].track(single, 7, None, turn=2, energy=energy, rest_energy=rest,
                               losses=True, strict=strict)
         np.testing.assert_array_equal(got, single)
-        np.testing.assert_array_equal(lms[i].flag.cpu().numpy().astype(bool), lm1["islost"])
-        np.testing.assert_array_equal(lms[i].turn.cpu().numpy().view(np.uint32), lm1["turn"])
-        np.testing.assert_array_equal(lms[i].elem.cpu().numpy().view(np.uint32), lm1["elem"])
+        np.testing.assert_array_equal(lms[i].flag.to_host().astype(bool), lm1["islost"])
+        np.testing.assert_array_equal(lms[i].turn.to_host(), lm1["turn"])
+        np.testing.assert_array_equal(lms[i].elem.to_host(), lm1["elem"])
         ref = p.copy(order="F")
         _, lmo = po.atpass(lat, ref, 7, turn=2, losses=True)
         np.testing.assert_array_equal(lm1["islost"], lmo["islost"])
@@ -57,25 +56,23 @@ def test_batch_equals_single_and_oracle(strict):
 
 
 def test_autotuner_picks_a_candidate_and_keeps_state(tmp_path, monkeypatch):
-    import torch
 
     monkeypatch.setenv("ATX_TUNE_CACHE", str(tmp_path / "tune.json"))
     lats, parts = _instances(cavity=True)
     dls = [engine.DeviceLattice(l) for l in lats]
-    dev = torch.device("cuda:0")
-    tens = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    tens = [DeviceArray.from_host(p.T) for p in parts]
     before = [t.clone() for t in tens]
     res = tune_batch(dls, tens, energy=lats[0].energy, rest_energy=lats[0].particle.rest_energy,
                      use_cache=False)
     assert res["block_size"] in (64, 128, 256, 512)
     assert set(res["ms"]) == {"64", "128", "256", "512"}
     for a, b in zip(tens, before):
-        assert torch.equal(a, b)
+        np.testing.assert_array_equal(a.to_host(), b.to_host())
 
 
 def _lm_arrays(lm):
-    return (lm.flag.cpu().numpy().astype(bool), lm.turn.cpu().numpy().view(np.uint32),
-            lm.elem.cpu().numpy().view(np.uint32), lm.coord.cpu().numpy())
+    return (lm.flag.to_host().astype(bool), lm.turn.to_host(),
+            lm.elem.to_host(), lm.coord.to_host())
 
 
 @pytest.mark.parametrize("losses", [True, False], ids=["losses", "nolosses"])
@@ -87,37 +84,35 @@ def test_repacked_launch_equals_plain(losses, block):
     counts that end in different blocks and an item (third of five) that
     loses every particle on its first turn, so that later blocks map no
     workgroup to it or to the finished items."""
-    import torch
 
     base = large_ring(ncells=2, cavity=True)
     lats = [error_realisation(base, 300 + i) for i in range(5)]
     dls = [engine.DeviceLattice(l) for l in lats]
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
-    dev = torch.device("cuda:0")
     sizes = [2000, 700, 600, 1500, 333]
     parts = [da_polar_grid(n, xmax=12e-3, ymax=5e-3) for n in sizes]
     gone = np.zeros((6, 600))
     gone[0] = np.linspace(0.05, 0.08, 600)  # outside the 10 mm apertures
     parts[2] = np.asfortranarray(gone)
     turns = [(37, 0), (64, 5), (40, 0), (9, 1), (1, 0)]
-    plain = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    plain = [DeviceArray.from_host(p.T) for p in parts]
     lm_plain = track_batch(dls, plain, 0, energy=energy, rest_energy=rest, losses=losses,
                            block_size=block, item_turns=turns)
-    torch.cuda.synchronize()
+    device.synchronize()
     nlost = 0
     for call in range(3):
-        rep = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+        rep = [DeviceArray.from_host(p.T) for p in parts]
         lm_rep = track_batch(dls, rep, 0, energy=energy, rest_energy=rest, losses=losses,
                              block_size=block, item_turns=turns, repack=True)
         # eager work between the graph runs
-        torch.cuda.synchronize()
+        device.synchronize()
         for i in range(len(parts)):
-            a, b = rep[i].cpu().numpy(), plain[i].cpu().numpy()
+            a, b = rep[i].to_host(), plain[i].to_host()
             np.testing.assert_array_equal(a, b, err_msg=f"call {call} item {i}")
             if losses:
                 for x, y in zip(_lm_arrays(lm_rep[i]), _lm_arrays(lm_plain[i])):
                     np.testing.assert_array_equal(x, y, err_msg=f"call {call} item {i}")
-                nlost += int(lm_rep[i].flag.sum())
+                nlost += int(lm_rep[i].flag.to_host().sum())
             else:
                 nlost += int(np.isnan(a[:, 0]).sum())
     assert nlost > 0
@@ -125,22 +120,20 @@ def test_repacked_launch_equals_plain(losses, block):
 
 def test_repacked_launch_continues_losses():
     """Two repacked calls with ATX_CONTINUE_LOSSES equal one plain call."""
-    import torch
 
     lats = [error_realisation(large_ring(ncells=2, cavity=True), 410 + i) for i in range(2)]
     dls = [engine.DeviceLattice(l) for l in lats]
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
-    dev = torch.device("cuda:0")
     parts = [da_polar_grid(900, xmax=12e-3, ymax=5e-3), da_polar_grid(450, xmax=12e-3, ymax=5e-3)]
-    one = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
+    one = [DeviceArray.from_host(p.T) for p in parts]
     lm_one = track_batch(dls, one, 20, energy=energy, rest_energy=rest)
-    two = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
-    lm_two = [LossMap.empty(p.shape[1], dev) for p in parts]
+    two = [DeviceArray.from_host(p.T) for p in parts]
+    lm_two = [LossMap.empty(p.shape[1]) for p in parts]
     track_batch(dls, two, 6, energy=energy, rest_energy=rest, loss_maps=lm_two, repack=True)
     track_batch(dls, two, 14, turn=6, energy=energy, rest_energy=rest, loss_maps=lm_two,
                 continue_losses=True, repack=True)
-    torch.cuda.synchronize()
+    device.synchronize()
     for i in range(2):
-        np.testing.assert_array_equal(two[i].cpu().numpy(), one[i].cpu().numpy())
+        np.testing.assert_array_equal(two[i].to_host(), one[i].to_host())
         for x, y in zip(_lm_arrays(lm_two[i]), _lm_arrays(lm_one[i])):
             np.testing.assert_array_equal(x, y)

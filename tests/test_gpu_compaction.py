@@ -10,7 +10,8 @@ when the loss map continues across calls."""
 import numpy as np
 import pytest
 
-from at_amd import engine
+from at_amd import device, engine
+from at_amd.device import DeviceArray
 from at_amd.batch import LossMap, track_batch
 from at_amd.workloads import da_polar_grid, error_realisation, large_ring
 from oracle import pyoracle as po
@@ -29,26 +30,24 @@ def _grid(n=3001):
 
 @pytest.mark.parametrize("block", [64, 128, 256, 512])
 def test_compacted_batch_equals_oracle(block):
-    import torch
 
     lat = _ring()
     dl = engine.DeviceLattice(lat)
     parts = _grid()
-    dev = torch.device("cuda:0")
-    t = torch.from_numpy(np.ascontiguousarray(parts.T)).to(dev)
+    t = DeviceArray.from_host(parts.T)
     (lm,) = track_batch([dl], [t], 12, energy=lat.energy, rest_energy=lat.particle.rest_energy,
                         block_size=block, strict=True)
-    torch.cuda.synchronize()
+    device.synchronize()
     ref = parts.copy(order="F")
     _, lmo = po.atpass(lat, ref, 12, losses=True)
     frac = lmo["islost"].mean()
     print(f"block {block}: lost fraction {frac:.3f}", flush=True)
     assert 0.2 < frac < 0.95
-    np.testing.assert_array_equal(t.cpu().numpy().T, ref)
-    np.testing.assert_array_equal(lm.flag.cpu().numpy().astype(bool), lmo["islost"])
-    np.testing.assert_array_equal(lm.turn.cpu().numpy().view(np.uint32), lmo["turn"])
-    np.testing.assert_array_equal(lm.elem.cpu().numpy().view(np.uint32), lmo["elem"])
-    np.testing.assert_array_equal(lm.coord.cpu().numpy().T, lmo["coord"])
+    np.testing.assert_array_equal(t.to_host().T, ref)
+    np.testing.assert_array_equal(lm.flag.to_host().astype(bool), lmo["islost"])
+    np.testing.assert_array_equal(lm.turn.to_host(), lmo["turn"])
+    np.testing.assert_array_equal(lm.elem.to_host(), lmo["elem"])
+    np.testing.assert_array_equal(lm.coord.to_host().T, lmo["coord"])
 
 
 def test_compacted_track_without_loss_map_equals_oracle():
@@ -81,22 +80,20 @@ def test_compacted_fast_build_keeps_loss_indices():
 
 
 def test_compaction_with_continued_loss_map():
-    import torch
 
     lat = _ring()
     dl = engine.DeviceLattice(lat)
     parts = _grid(1500)
-    dev = torch.device("cuda:0")
-    t = torch.from_numpy(np.ascontiguousarray(parts.T)).to(dev)
-    lm = LossMap.empty(parts.shape[1], dev)
+    t = DeviceArray.from_host(parts.T)
+    lm = LossMap.empty(parts.shape[1])
     e, rest = lat.energy, lat.particle.rest_energy
     for t0, nt in ((0, 4), (4, 5), (9, 3)):
         track_batch([dl], [t], 0, energy=e, rest_energy=rest, loss_maps=[lm],
                     item_turns=[(nt, t0)], continue_losses=True, strict=True)
-    torch.cuda.synchronize()
+    device.synchronize()
     ref = parts.copy(order="F")
     _, lmo = po.atpass(lat, ref, 12, losses=True)
-    np.testing.assert_array_equal(t.cpu().numpy().T, ref)
-    np.testing.assert_array_equal(lm.flag.cpu().numpy().astype(bool), lmo["islost"])
-    np.testing.assert_array_equal(lm.turn.cpu().numpy().view(np.uint32), lmo["turn"])
-    np.testing.assert_array_equal(lm.elem.cpu().numpy().view(np.uint32), lmo["elem"])
+    np.testing.assert_array_equal(t.to_host().T, ref)
+    np.testing.assert_array_equal(lm.flag.to_host().astype(bool), lmo["islost"])
+    np.testing.assert_array_equal(lm.turn.to_host(), lmo["turn"])
+    np.testing.assert_array_equal(lm.elem.to_host(), lmo["elem"])

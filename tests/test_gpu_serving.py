@@ -10,7 +10,8 @@ continued loss map are checked against single launches too."""
 import numpy as np
 import pytest
 
-from at_amd import engine
+from at_amd import device, engine
+from at_amd.device import DeviceArray
 from at_amd.batch import LossMap, track_batch
 from at_amd.serving import Request, Server, serve_one_by_one
 from at_amd.workloads import da_polar_grid, error_realisation, gaussian_beam, large_ring
@@ -102,29 +103,27 @@ def test_strict_served_request_equals_oracle():
 
 
 def test_batch_item_turns_and_continued_losses():
-    import torch
 
     lats = _instances(2)
     energy, rest = lats[0].energy, lats[0].particle.rest_energy
     dls = [engine.DeviceLattice(l) for l in lats]
-    dev = torch.device("cuda:0")
     parts = [da_polar_grid(700, xmax=12e-3, ymax=5e-3), da_polar_grid(333, xmax=12e-3, ymax=5e-3)]
-    tens = [torch.from_numpy(np.ascontiguousarray(p.T)).to(dev) for p in parts]
-    lms = [LossMap.empty(p.shape[1], dev) for p in parts]
+    tens = [DeviceArray.from_host(p.T) for p in parts]
+    lms = [LossMap.empty(p.shape[1]) for p in parts]
     # instance 0: turns 0-2 then 3-9; instance 1: turns 0-6 then nothing
     track_batch(dls, tens, 0, energy=energy, rest_energy=rest, loss_maps=lms,
                 item_turns=[(3, 0), (7, 0)], continue_losses=True)
     track_batch(dls, tens, 0, energy=energy, rest_energy=rest, loss_maps=lms,
                 item_turns=[(7, 3), (0, 7)], continue_losses=True)
-    torch.cuda.synchronize()
+    device.synchronize()
     for i, nt in ((0, 10), (1, 7)):
         r = np.asfortranarray(parts[i].copy())
         _, lm = dls[i].track(r, nt, None, energy=energy, rest_energy=rest, losses=True)
-        np.testing.assert_array_equal(tens[i].cpu().numpy().T, r)
-        np.testing.assert_array_equal(lms[i].flag.cpu().numpy().astype(bool), lm["islost"])
-        np.testing.assert_array_equal(lms[i].turn.cpu().numpy().view(np.uint32), lm["turn"])
-        np.testing.assert_array_equal(lms[i].elem.cpu().numpy().view(np.uint32), lm["elem"])
-        np.testing.assert_array_equal(lms[i].coord.cpu().numpy().T, lm["coord"])
+        np.testing.assert_array_equal(tens[i].to_host().T, r)
+        np.testing.assert_array_equal(lms[i].flag.to_host().astype(bool), lm["islost"])
+        np.testing.assert_array_equal(lms[i].turn.to_host(), lm["turn"])
+        np.testing.assert_array_equal(lms[i].elem.to_host(), lm["elem"])
+        np.testing.assert_array_equal(lms[i].coord.to_host().T, lm["coord"])
         assert lm["islost"].any()
 
 

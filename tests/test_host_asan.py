@@ -19,8 +19,11 @@ HIPCC = "/opt/rocm/bin/hipcc"
 @pytest.mark.skipif(not os.path.exists(HIPCC) or shutil.which("g++") is None,
                     reason="hipcc and g++ are needed")
 def test_host_abi_under_address_sanitizer(tmp_path):
-    objs = [os.path.join(BUILD, f) for f in ("atx_kernel_fast.o", "atx_kernel_strict.o",
-                                               "atx_adjoint.o")]
+    from at_amd.build import SHAPES
+
+    objs = [os.path.join(BUILD, f"atx_kernel_{b}_p{p}_u{u}.o")
+            for b in ("strict", "fast") for p, u in SHAPES]
+    objs.append(os.path.join(BUILD, "atx_adjoint.o"))
     if not all(os.path.exists(o) for o in objs):
         pytest.skip("device objects not built (python -m at_amd.build)")
     host = str(tmp_path / "host.o")
