@@ -139,10 +139,10 @@ def bench_batch(args) -> dict:
     base = [device.DeviceArray.from_host(p.T) for _, p in work]
     for dl in dls:
         dl.check(energy, rest)
-    tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False,
-                      repack=not args.no_repack)
-    shape = {k: tune[k] for k in ("block_size", "ppt", "unroll") if k in tune}
     T = args.batch_turns
+    tune = tune_batch(dls, base, energy=energy, rest_energy=rest, use_cache=False,
+                      repack=not args.no_repack, nturns=T)
+    shape = {k: tune[k] for k in ("block_size", "ppt", "unroll")}
     print(f"batch: autotuned {shape}; {len(dls)} instances, {T} turns",
           file=sys.stderr, flush=True)
     stream = device.Stream()
@@ -206,7 +206,8 @@ def bench_batch(args) -> dict:
                                f"particles ({ntot} total), {T} turns, one launch"
                                + ("" if args.no_repack else
                                   " (turn blocks with survivors repacked, one graph)"),
-                   "launch_shape": shape, "autotune_ms": tune["ms"], "wall_s": wall},
+                   "launch_shape": shape, "autotune_model_ms": tune["ms"],
+                   "autotune_pilot_turns": tune["pilot_turns"], "wall_s": wall},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
         "plain_launch_ms": plain_ms,

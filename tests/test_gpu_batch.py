@@ -63,9 +63,12 @@ def test_autotuner_picks_a_candidate_and_keeps_state(tmp_path, monkeypatch):
     tens = [DeviceArray.from_host(p.T) for p in parts]
     before = [t.clone() for t in tens]
     res = tune_batch(dls, tens, energy=lats[0].energy, rest_energy=lats[0].particle.rest_energy,
-                     use_cache=False)
+                     use_cache=False, nturns=40)
     assert res["block_size"] in (64, 128, 256, 512)
-    assert set(res["ms"]) == {"64", "128", "256", "512"}
+    assert (res["ppt"], res["unroll"]) in {(1, 1), (1, 2), (1, 4), (2, 1), (2, 2)}
+    # staged search: the five kernels at 256 lanes, then the other sizes
+    assert len(res["ms"]) == 8 and f"b{res['block_size']}p{res['ppt']}u{res['unroll']}" in res["ms"]
+    assert res["ms"][f"b{res['block_size']}p{res['ppt']}u{res['unroll']}"] == min(res["ms"].values())
     for a, b in zip(tens, before):
         np.testing.assert_array_equal(a.to_host(), b.to_host())
 
