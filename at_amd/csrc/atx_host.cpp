@@ -1059,6 +1059,53 @@ int check_partial(int nelem, int nturns, int& start, int& end) {
 
 using namespace atx;
 
+// --- Instance table of atx_track_batch_noalloc --------------------------------
+// A file-scope device array, one per GPU, holding the instance table of the
+// last zero-allocation launch on that GPU. In host code `scratch` only names
+// the symbol: its device address is taken once per device with
+// hipGetSymbolAddress and used from then on.
+constexpr int kNoAllocItems = 64;
+static __device__ __align__(256) Instance scratch[kNoAllocItems];
+
+namespace atx {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+Instance* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (first call on a device:
+// hipGetSymbolAddress, with that device current).
+int scratch_address(Instance** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ATX_E_RUNTIME, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ATX_E_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<Instance*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return ATX_OK;
+}
+
+// Instances travel to the device as kernel arguments, 16 per launch.
+constexpr int kChunk = 16;
+struct InstanceChunk {
+  Instance v[kChunk];
+};
+
+__global__ void __launch_bounds__(64) put_instances(Instance* dst, InstanceChunk c, int n) {
+  const int i = (int)threadIdx.x;
+  if (i < n) dst[i] = c.v[i];
+}
+
+}  // namespace
+}  // namespace atx
+
 extern "C" {
 
 const char* atx_last_error(void) { return g_err.c_str(); }
@@ -1358,6 +1405,123 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
   if (flags & ATX_REPACK)
     return launch_repacked(sorted_insts, sorted_slots, shape, stream, strict, ext);
   return launch_instances(sorted_insts, sorted_slots, shape, stream, strict, ext);
+}
+
+int atx_track_batch_noalloc(const atx_batch_item* items, int32_t nitems, int32_t nturns,
+                            int32_t turn, double energy, double rest_energy, int32_t losses,
+                            const atx_launch_shape* shape_in, void* stream_, uint32_t flags) {
+  g_err.clear();
+  if (nitems < 0 || (nitems > 0 && !items)) return fail(ATX_E_VALUE, "bad item list");
+  if (nitems > kNoAllocItems)
+    return fail(ATX_E_VALUE, "atx_track_batch_noalloc takes at most 64 items");
+  if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
+  if (flags & ATX_REPACK)
+    return fail(ATX_E_VALUE, "ATX_REPACK needs working memory: not available without allocation");
+  atx_launch_shape shape;
+  if (int rc = resolve_shape(shape_in, shape)) return rc;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  Instance* d_tab = nullptr;
+  if (int rc = scratch_address(&d_tab)) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool strict = (flags & ATX_STRICT) != 0;
+  const bool soa = (flags & ATX_SOA) != 0;
+  for (int i = 0; i < nitems; i++) {
+    const atx_batch_item& it = items[i];
+    if (!it.lattice) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL lattice");
+    if (it.num_particles > 0 && !it.r) return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": NULL r");
+    if (losses && it.num_particles > 0 && (!it.lost_flag || !it.lost_turn || !it.lost_elem || !it.lost_coord))
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": loss arrays are NULL");
+    if (it.nturns >= 0 && it.turn < 0)
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) + ": turn must be >= 0");
+  }
+  // each distinct lattice locked once, in address order (as atx_track_batch)
+  atx_lattice* distinct[kNoAllocItems];
+  int nd = 0;
+  for (int i = 0; i < nitems; i++) distinct[nd++] = items[i].lattice;
+  std::sort(distinct, distinct + nd, std::less<atx_lattice*>());
+  nd = (int)(std::unique(distinct, distinct + nd) - distinct);
+  struct Locks {
+    atx_lattice** l;
+    int n = 0;
+    ~Locks() {
+      for (int i = n - 1; i >= 0; i--) l[i]->mu.unlock();
+    }
+  } held{distinct};
+  for (int i = 0; i < nd; i++) {
+    distinct[i]->mu.lock();
+    held.n = i + 1;
+  }
+  Instance tab[kNoAllocItems];
+  int64_t work[kNoAllocItems];
+  int n = 0;
+  bool ext = false;
+  for (int i = 0; i < nitems; i++) {
+    const atx_batch_item& it = items[i];
+    atx_lattice* lat = it.lattice;
+    // compiling or uploading a lattice allocates: it must already be on
+    // this device for this energy
+    if (!(lat->dev_valid && lat->dev_energy == energy && lat->dev_rest == rest_energy &&
+          !lat->dev_elempass && lat->dev_device == dev))
+      return fail(ATX_E_VALUE, "item " + std::to_string(i) +
+                                   ": lattice not on this device for this energy (track it once "
+                                   "with atx_track or atx_track_batch first)");
+    const int nt = it.nturns >= 0 ? it.nturns : nturns;
+    if (it.num_particles == 0 || nt == 0) continue;
+    ext |= lat->has_ext;
+    Instance& in = tab[n];
+    std::memset(&in, 0, sizeof(in));
+    const int nelem = (int)lat->elems.size();
+    in.elems = lat->d_elems;
+    in.pool = lat->d_pool;
+    in.slots = nullptr;  // no refpts: the kernel reads no slot table
+    in.r = it.r;
+    in.lost_flag = it.lost_flag;
+    in.lost_turn = it.lost_turn;
+    in.lost_elem = it.lost_elem;
+    in.lost_coord = it.lost_coord;
+    in.pstride = soa ? 1 : 6;
+    in.cstride = soa ? (int64_t)it.num_particles : 1;
+    in.nelem = nelem;
+    in.npart = (int32_t)it.num_particles;
+    in.nturns = nt;
+    in.turn0 = it.nturns >= 0 ? it.turn : turn;
+    in.start_elem = 0;
+    in.end_elem = nelem;
+    in.losses = losses ? ((flags & ATX_CONTINUE_LOSSES) ? 2 : 1) : 0;
+    work[n] = (int64_t)nt * nelem;
+    n++;
+  }
+  if (n == 0) return ATX_OK;
+  // longest work first (insertion sort: stable, in place)
+  for (int i = 1; i < n; i++) {
+    const Instance v = tab[i];
+    const int64_t w = work[i];
+    int j = i - 1;
+    for (; j >= 0 && work[j] < w; j--) {
+      tab[j + 1] = tab[j];
+      work[j + 1] = work[j];
+    }
+    tab[j + 1] = v;
+    work[j + 1] = w;
+  }
+  const int per_wg = shape.block * shape.ppt;
+  int nblocks = 0;
+  for (int i = 0; i < n; i++) {
+    tab[i].block_begin = nblocks;
+    nblocks += (int)((tab[i].npart + per_wg - 1) / per_wg);
+  }
+  for (int c = 0; c < n; c += kChunk) {
+    InstanceChunk chunk;
+    const int m = std::min(kChunk, n - c);
+    for (int k = 0; k < m; k++) chunk.v[k] = tab[c + k];
+    hipLaunchKernelGGL(put_instances, dim3(1), dim3(64), 0, stream, d_tab + c, chunk, m);
+    HIPCHK(hipGetLastError());
+  }
+  const hipError_t le = track_launcher(strict, shape)(d_tab, n, nblocks, shape.block, ext ? 1 : 0,
+                                                      nullptr, stream);
+  if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
+  return ATX_OK;
 }
 
 int atx_lattice_set_launch_shape(atx_lattice* lat, const atx_launch_shape* shape) {

@@ -45,6 +45,7 @@ EXPORTS = (
     "atx_stream_create", "atx_stream_destroy", "atx_stream_synchronize", "atx_stream_wait_event",
     "atx_event_create", "atx_event_destroy", "atx_event_record", "atx_event_query",
     "atx_event_synchronize", "atx_event_elapsed", "atx_lattice_set_launch_shape",
+    "atx_track_batch_noalloc",
 )
 
 
@@ -145,6 +146,7 @@ def load_library(path: str | None = None) -> C.CDLL:
                                         C.c_double, C.c_double, C.c_int32,
                                         C.POINTER(LaunchShape), C.c_void_p, C.c_uint32]
         lib.atx_lattice_set_launch_shape.argtypes = [C.c_void_p, C.POINTER(LaunchShape)]
+        lib.atx_track_batch_noalloc.argtypes = lib.atx_track_batch.argtypes
         lib.atx_lattice_turn_counter.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.atx_lattice_set_integrator.argtypes = [C.c_void_p, C.c_int32]
         lib.atx_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint32,
@@ -401,7 +403,8 @@ class DeviceLattice:
 def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
                 rest_energy: float = 0.0, losses: bool = True, block_size: int = 0,
                 ppt: int = 0, unroll: int = 0, stream: int | None = None, strict: bool = False,
-                soa: bool = False, continue_losses: bool = False, repack: bool = False) -> None:
+                soa: bool = False, continue_losses: bool = False, repack: bool = False,
+                noalloc: bool = False) -> None:
     """One launch over many (DeviceLattice, device particle buffer) pairs.
 
     items: sequence of dicts with keys ``lattice`` (DeviceLattice), ``r``
@@ -409,7 +412,9 @@ def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
     turn, elem, coord) device pointers, and optionally ``nturns``/``turn``
     for an item that tracks its own number of turns. ``repack``: turn blocks
     with the survivors packed between them, run as one graph (ATX_REPACK).
-    block_size, ppt, unroll: the launch shape (0: the engine's default)."""
+    block_size, ppt, unroll: the launch shape (0: the engine's default).
+    noalloc: through atx_track_batch_noalloc (no allocation; at most 64
+    items, no repack, lattices already on the device)."""
     lib = load_library()
     arr = (BatchItem * max(len(items), 1))()
     for i, it in enumerate(items):
@@ -424,9 +429,9 @@ def track_batch(items: list, nturns: int, *, turn: int = 0, energy: float = 0.0,
     flags = ((ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0) | ATX_DEVICE_POINTERS
              | (ATX_CONTINUE_LOSSES if continue_losses else 0) | (ATX_REPACK if repack else 0))
     sh = launch_shape(block_size, ppt, unroll)
-    _check(lib.atx_track_batch(arr, len(items), int(nturns), int(turn), float(energy),
-                               float(rest_energy), int(bool(losses)),
-                               None if sh is None else C.byref(sh), stream, flags), lib)
+    fn = lib.atx_track_batch_noalloc if noalloc else lib.atx_track_batch
+    _check(fn(arr, len(items), int(nturns), int(turn), float(energy), float(rest_energy),
+              int(bool(losses)), None if sh is None else C.byref(sh), stream, flags), lib)
 
 
 def device_count() -> int:

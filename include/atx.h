@@ -161,6 +161,21 @@ int atx_track_batch(const atx_batch_item *items, int32_t num_items, int32_t ntur
                     double energy, double rest_energy, int32_t losses,
                     const atx_launch_shape *shape, void *stream, uint32_t flags);
 
+/* atx_track_batch without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as atx_track_batch, with these limits: at most 64
+ * items; no ATX_REPACK (its survivor lists need working memory); every
+ * item's lattice already resident on the current device for this energy
+ * and rest energy (track it once with atx_track or atx_track_batch first).
+ * The instance table lives in a fixed device array of the library, one per
+ * GPU (64 entries, about 10 KB), written by each call: calls that overlap on
+ * one GPU (on different streams, or from several threads) race on it. Calls
+ * no hipMalloc, hipFree or other allocating routine. */
+int atx_track_batch_noalloc(const atx_batch_item *items, int32_t num_items, int32_t nturns,
+                            int32_t turn, double energy, double rest_energy, int32_t losses,
+                            const atx_launch_shape *shape, void *stream, uint32_t flags);
+
 /* --- Differentiation -----------------------------------------------------
  * Reverse-mode gradient through nturns turns of the whole ring. No
  * counterpart in the reference, whose tangent maps (diff_drift.c,
