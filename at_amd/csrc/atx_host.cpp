@@ -682,7 +682,11 @@ int ensure_device_lattice(atx_lattice* lat, double energy, double rest, bool ele
 
 constexpr int kBlock = 256;
 constexpr int kPpt = 1;
-constexpr int kUnroll = 2;
+// two slices per loop iteration in the fast build (120 VGPRs, four waves per
+// SIMD); one in the strict build, whose two-slice kernel needs 136 VGPRs and
+// would run three waves per SIMD
+constexpr int kUnrollFast = 2;
+constexpr int kUnrollStrict = 1;
 
 typedef hipError_t (*TrackLaunch)(const Instance*, int, int, int, int, const int32_t*, hipStream_t);
 
@@ -698,8 +702,8 @@ TrackLaunch track_launcher(bool strict, const atx_launch_shape& sh) {
 }
 
 // Launch shape with the defaults filled in, or an error for an invalid one.
-int resolve_shape(const atx_launch_shape* in, atx_launch_shape& out) {
-  out = atx_launch_shape{kBlock, kPpt, kUnroll, 0};
+int resolve_shape(const atx_launch_shape* in, atx_launch_shape& out, bool strict) {
+  out = atx_launch_shape{kBlock, kPpt, strict ? kUnrollStrict : kUnrollFast, 0};
   if (in) {
     if (in->block) out.block = in->block;
     if (in->ppt) out.ppt = in->ppt;
@@ -1020,7 +1024,7 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
   if (rc) return rc;
   if (np == 0 || nturns <= 0) return ATX_OK;
   atx_launch_shape shape;
-  rc = resolve_shape(&lat->shape, shape);
+  rc = resolve_shape(&lat->shape, shape, strict);
   if (rc) return rc;
   // Without a shape set on the lattice, beams of two or more 512-lane
   // workgroups per CU run in 512-lane workgroups: survivors are packed
@@ -1327,10 +1331,10 @@ int atx_track_batch(const atx_batch_item* items, int32_t nitems, int32_t nturns,
   g_err.clear();
   if (nitems < 0 || (nitems > 0 && !items)) return fail(ATX_E_VALUE, "bad item list");
   if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
-  atx_launch_shape shape;
-  if (int rc = resolve_shape(shape_in, shape)) return rc;
-  hipStream_t stream = (hipStream_t)stream_;
   const bool strict = (flags & ATX_STRICT) != 0;
+  atx_launch_shape shape;
+  if (int rc = resolve_shape(shape_in, shape, strict)) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
   const bool soa = (flags & ATX_SOA) != 0;
   for (int i = 0; i < nitems; i++)
     if (items[i].nturns >= 0 && items[i].turn < 0)
@@ -1417,14 +1421,14 @@ int atx_track_batch_noalloc(const atx_batch_item* items, int32_t nitems, int32_t
   if (nturns < 0) return fail(ATX_E_VALUE, "nturns must be >= 0");
   if (flags & ATX_REPACK)
     return fail(ATX_E_VALUE, "ATX_REPACK needs working memory: not available without allocation");
+  const bool strict = (flags & ATX_STRICT) != 0;
   atx_launch_shape shape;
-  if (int rc = resolve_shape(shape_in, shape)) return rc;
+  if (int rc = resolve_shape(shape_in, shape, strict)) return rc;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   Instance* d_tab = nullptr;
   if (int rc = scratch_address(&d_tab)) return rc;
   hipStream_t stream = (hipStream_t)stream_;
-  const bool strict = (flags & ATX_STRICT) != 0;
   const bool soa = (flags & ATX_SOA) != 0;
   for (int i = 0; i < nitems; i++) {
     const atx_batch_item& it = items[i];
@@ -1527,7 +1531,7 @@ int atx_track_batch_noalloc(const atx_batch_item* items, int32_t nitems, int32_t
 int atx_lattice_set_launch_shape(atx_lattice* lat, const atx_launch_shape* shape) {
   if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
   atx_launch_shape s;
-  if (int rc = resolve_shape(shape, s)) return rc;
+  if (int rc = resolve_shape(shape, s, false)) return rc;
   std::lock_guard<std::mutex> g(lat->mu);
   lat->shape = shape ? *shape : atx_launch_shape{0, 0, 0, 0};
   return ATX_OK;
