@@ -578,15 +578,17 @@ __device__ __forceinline__ void rad_str_kick_ang(R6& r, Ang& g, cdouble* a, cdou
   const double opd = 1.0 + r.dp;
   const double cr = __builtin_fma(im, g.yp, -re * g.xp);
   const double b2 = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im));
-  double f = __builtin_fma(-0.5 * s2, __builtin_fma(s2, s2, 1.0 - s2), 1.0);
+  // 1 - (s2/2)(1 - s2 + s2^2) by Horner in s2
+  double f = __builtin_fma(s2, __builtin_fma(s2, __builtin_fma(-0.5, s2, 0.5), -0.5), 1.0);
   const bool wide = s2 > 1e-4;
   if (__builtin_expect(__ballot(wide) != 0, 0)) {
     if (wide) f = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);
   }
   r.dp = __builtin_fma(-radk, (b2 * f) * (opd * opd), r.dp);
   const double pn = rcp_after_kick(1.0 + r.dp, g.pn);
-  g.xp = __builtin_fma(-kl * re, pn, g.xp);
-  g.yp = __builtin_fma(kl * im, pn, g.yp);
+  const double klpn = kl * pn;
+  g.xp = __builtin_fma(-re, klpn, g.xp);
+  g.yp = __builtin_fma(im, klpn, g.yp);
   g.pn = pn;
 }
 
@@ -612,8 +614,9 @@ __device__ __forceinline__ void rad_bnd_kick_ang(R6& r, Ang& g, cdouble* a, cdou
   const double pn = rcp_after_kick(1.0 + r.dp, g.pn);
   const double u = __builtin_fma(-x, irho, d0);
   const double v = __builtin_fma(-u, irho, re);
-  g.xp = __builtin_fma(-kl * v, pn, g.xp);
-  g.yp = __builtin_fma(kl * im, pn, g.yp);
+  const double klpn = kl * pn;
+  g.xp = __builtin_fma(-v, klpn, g.xp);
+  g.yp = __builtin_fma(im, klpn, g.yp);
   r.ct = __builtin_fma(kli, x, r.ct);
   g.pn = pn;
 }
