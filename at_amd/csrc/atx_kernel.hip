@@ -260,14 +260,14 @@ __device__ __forceinline__ double rcp1(double d) {
 }
 
 // Fast build: reciprocal of 1 + delta after a kick from the one before it,
-// pn = 1/(1 + delta_before). The radiated fraction e = 1 - (1 + delta) pn
-// is about 1e-7 per kick; two Newton steps leave e^4 + an ulp. A wave in
-// which any lane radiated more than 1e-4 in one kick takes fast_rcp.
+// pn = 1/(1 + delta_before). With the radiated fraction e = 1 - (1 + delta) pn
+// (about 1e-7 per kick), 1/(1 + delta) = pn/(1 - e) = pn (1 + e + e^2) to a
+// relative e^3, below 1e-17 for |e| <= 2e-6. A wave in which any lane
+// radiated more than that in one kick takes fast_rcp.
 __device__ __forceinline__ double rcp_after_kick(double opd2, double pn) {
   const double e = __builtin_fma(-opd2, pn, 1.0);
-  double q = __builtin_fma(pn, e, pn);
-  q = __builtin_fma(q, __builtin_fma(-opd2, q, 1.0), q);
-  const bool far = __builtin_fabs(e) > 1e-4;
+  double q = __builtin_fma(pn, __builtin_fma(e, e, e), pn);
+  const bool far = __builtin_fabs(e) > 2e-6;
   if (__builtin_expect(__ballot(far) != 0, 0)) {
     if (far) q = fast_rcp(opd2);
   }
