@@ -23,8 +23,8 @@ from .acceptance import (  # noqa: E402,F401
     get_momentum_acceptance, get_vertical_acceptance,
 )
 from .physics import (  # noqa: E402,F401
-    DConstant, ELossMethod, find_elem_m66, find_m44, find_m66, find_orbit, find_orbit4,
-    find_orbit6, find_sync_orbit, get_energy_loss, get_timelag_fromU0,
+    DConstant, ELossMethod, element_matrices, find_elem_m66, find_m44, find_m66, find_orbit,
+    find_orbit4, find_orbit6, find_sync_orbit, get_energy_loss, get_timelag_fromU0,
 )
 
 # Lattice methods, as pyat attaches them (physics/__init__.py, acceptance)

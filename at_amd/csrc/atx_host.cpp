@@ -35,6 +35,8 @@ ATX_TRACK_DECL(atx_launch_track_fast_p2_u1)
 ATX_TRACK_DECL(atx_launch_track_fast_p2_u2)
 #undef ATX_TRACK_DECL
 extern "C" hipError_t atx_launch_diffusion(const atx::DiffArgs*, hipStream_t);
+extern "C" hipError_t atx_launch_element_probes(const double*, double, double*, int, hipStream_t);
+extern "C" hipError_t atx_launch_element_diffs(const double*, double, double*, int, hipStream_t);
 extern "C" hipError_t atx_launch_repack(const atx::Instance*, int, int, hipStream_t);
 extern "C" hipError_t atx_launch_live_map(const atx::Instance*, int, int, int, int, int32_t*,
                                           hipStream_t);
@@ -1712,6 +1714,64 @@ int atx_diffusion_matrices(atx_lattice* lat, const double* orbits, double* bdiff
   hipError_t se = hipStreamSynchronize(stream);
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("diffusion launch: ") + hipGetErrorString(le));
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipMemcpyAsync: ") + hipGetErrorString(ce));
+  if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+  if (se != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+  return ATX_OK;
+}
+
+int atx_element_matrices(atx_lattice* lat, const double* orbits, double step, double* m66,
+                         double energy, double rest_energy, void* stream_, uint32_t flags) {
+  g_err.clear();
+  if (!lat) return fail(ATX_E_VALUE, "NULL lattice");
+  std::lock_guard<std::mutex> g(lat->mu);
+  const int nelem = (int)lat->elems.size();
+  if (nelem == 0) return ATX_OK;
+  if (!orbits || !m66) return fail(ATX_E_VALUE, "orbits/m66 are NULL");
+  if (!(step != 0.0) || !std::isfinite(step)) return fail(ATX_E_VALUE, "step must be finite and nonzero");
+  int rc = ensure_device_lattice(lat, energy, rest_energy, true);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool dev = (flags & ATX_DEVICE_POINTERS) != 0;
+  const bool strict = (flags & ATX_STRICT) != 0;
+  // probes (6, 12 nelem), then (host arrays) the orbits and the matrices
+  const size_t nr = 72 * (size_t)nelem, no = 6 * (size_t)nelem, nm = 36 * (size_t)nelem;
+  double* d_r = nullptr;
+  HIPCHK(hipMallocAsync((void**)&d_r, (nr + (dev ? 0 : no + nm)) * sizeof(double), stream));
+  const double* d_o = dev ? orbits : d_r + nr;
+  double* d_m = dev ? m66 : d_r + nr + no;
+  // One instance per element: its 12 probes through that element alone
+  // (at_elempass: no loss check), one 64-lane workgroup each.
+  std::vector<int32_t> slots(2 * (size_t)(nelem + 1));
+  std::vector<Instance> insts(nelem);
+  rc = single_instance(lat, d_r, 6, 1, 12, 1, nullptr, 0, 0, 0, 1, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, 0, true, slots.data(), insts[0]);
+  if (rc) {
+    (void)hipFreeAsync(d_r, stream);
+    return rc;
+  }
+  for (int e = 1; e < nelem; e++) {
+    insts[e] = insts[0];
+    insts[e].r = d_r + 72 * (size_t)e;
+    insts[e].start_elem = e;
+    insts[e].end_elem = e + 1;
+  }
+  const atx_launch_shape in{64, 1, 0, 0};
+  atx_launch_shape shape;
+  rc = resolve_shape(&in, shape, strict);
+  hipError_t e = hipSuccess;
+  if (!dev) e = hipMemcpyAsync(const_cast<double*>(d_o), orbits, no * sizeof(double),
+                               hipMemcpyHostToDevice, stream);
+  if (!rc && e == hipSuccess) e = atx_launch_element_probes(d_o, step, d_r, nelem, stream);
+  if (!rc && e == hipSuccess)
+    rc = launch_instances(insts, std::vector<const std::vector<int32_t>*>(nelem, &slots), shape,
+                          stream, strict, lat->has_ext);
+  if (!rc && e == hipSuccess) e = atx_launch_element_diffs(d_r, step, d_m, nelem, stream);
+  if (!rc && e == hipSuccess && !dev)
+    e = hipMemcpyAsync(m66, d_m, nm * sizeof(double), hipMemcpyDeviceToHost, stream);
+  hipError_t fe = hipFreeAsync(d_r, stream);
+  hipError_t se = dev ? hipSuccess : hipStreamSynchronize(stream);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ATX_E_RUNTIME, std::string("element matrices: ") + hipGetErrorString(e));
   if (fe != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
   if (se != hipSuccess) return fail(ATX_E_RUNTIME, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
   return ATX_OK;

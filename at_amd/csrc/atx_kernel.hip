@@ -1768,6 +1768,37 @@ extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nb
 
 #if ATX_AUX
 namespace atx {
+// Probes of atx_element_matrices (matrix.py:195-200): particle 12 e + c is
+// the entrance orbit of element e plus column c of [D, -D], D = diag(step/2):
+// +step/2 on coordinate c (c < 6), -step/2 on coordinate c - 6 (c >= 6), and
+// +0.0 (c < 6) or -0.0 (c >= 6) elsewhere, as numpy adds them; (6, 12 nelem)
+// F-order.
+__global__ void __launch_bounds__(256) element_probe_kernel(const double* __restrict__ orbits,
+                                                            double step, double* __restrict__ r,
+                                                            int nelem) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 12 * (int64_t)nelem) return;
+  const int64_t e = i / 12;
+  const int c = (int)(i % 12);
+  const double d = (c < 6) ? 0.5 * step : -(0.5 * step);
+  const double z = (c < 6) ? 0.0 : -0.0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) r[6 * i + k] = orbits[6 * e + k] + ((k == c % 6) ? d : z);
+}
+
+// m[i][j] = (probe j+ - probe j-)_i / step (matrix.py:201-202), column-major
+// 6x6 per element ([i][j] at 36 e + i + 6 j).
+__global__ void __launch_bounds__(256) element_diff_kernel(const double* __restrict__ r,
+                                                           double step, double* __restrict__ m66,
+                                                           int nelem) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 36 * (int64_t)nelem) return;
+  const int64_t e = t / 36;
+  const int i = (int)(t % 6), j = (int)((t % 36) / 6);
+  const double* p = r + 72 * e;
+  m66[t] = (p[6 * j + i] - p[6 * (6 + j) + i]) / step;
+}
+
 // Repack between the turn blocks of a batched launch: one workgroup per
 // instance lists, in order, the particles still alive (x not NaN: a lost
 // particle is set to NaN, at.c:142-161) in idx[0..*nlive). The first pass
@@ -1892,6 +1923,24 @@ extern "C" hipError_t atx_launch_live_map(const atx::Instance* d_insts, int nins
   if (ninst <= 0) return hipSuccess;
   hipLaunchKernelGGL(atx::live_map_kernel, dim3(1), dim3(1024), 0, stream, d_insts, ninst, block,
                      fine, fine_limit, live_begin);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t atx_launch_element_probes(const double* orbits, double step, double* r,
+                                                int nelem, hipStream_t stream) {
+  const int64_t n = 12 * (int64_t)nelem;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::element_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream, orbits, step, r, nelem);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t atx_launch_element_diffs(const double* r, double step, double* m66,
+                                               int nelem, hipStream_t stream) {
+  const int64_t n = 36 * (int64_t)nelem;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(atx::element_diff_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream, r, step, m66, nelem);
   return hipGetLastError();
 }
 

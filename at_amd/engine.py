@@ -40,7 +40,7 @@ EXPORTS = (
     "atx_lattice_num_elements", "atx_lattice_check", "atx_track", "atx_elempass",
     "atx_track_batch", "atx_lattice_turn_counter", "atx_device_count", "atx_device_info",
     "atx_set_device", "atx_lattice_set_integrator", "atx_adjoint", "atx_element_set_array_at",
-    "atx_diffusion_matrices", "atx_adjoint_batch", "atx_shutdown", "atx_get_device",
+    "atx_diffusion_matrices", "atx_element_matrices", "atx_adjoint_batch", "atx_shutdown", "atx_get_device",
     "atx_device_alloc", "atx_device_free", "atx_host_alloc", "atx_host_free", "atx_memcpy",
     "atx_stream_create", "atx_stream_destroy", "atx_stream_synchronize", "atx_stream_wait_event",
     "atx_event_create", "atx_event_destroy", "atx_event_record", "atx_event_query",
@@ -159,6 +159,8 @@ def load_library(path: str | None = None) -> C.CDLL:
                                           C.c_void_p]
         lib.atx_element_set_array_at.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, _dp,
                                                  C.c_int64]
+        lib.atx_element_matrices.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                             C.c_double, C.c_double, C.c_void_p, C.c_uint32]
         lib.atx_diffusion_matrices.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                                C.c_double, C.c_void_p, C.c_uint32]
         lib.atx_device_count.argtypes = [C.POINTER(C.c_int)]
@@ -330,6 +332,20 @@ class DeviceLattice:
                                                float(energy), float(rest_energy), None, 0),
                self.lib)
         return b.reshape(self.nelem, 6, 6).transpose(0, 2, 1).copy()
+
+    def element_matrices(self, orbits: np.ndarray, step: float, energy: float,
+                         rest_energy: float, strict: bool = False) -> np.ndarray:
+        """6x6 matrix of every element around its entrance orbit by central
+        differences of step (atx_element_matrices): orbits (6, nelem) ->
+        (nelem, 6, 6)."""
+        o = np.asfortranarray(orbits, dtype=np.float64)
+        if o.shape != (6, self.nelem):
+            raise ValueError("orbits must have shape (6, number of elements)")
+        m = np.zeros(36 * self.nelem, dtype=np.float64)
+        _check(self.lib.atx_element_matrices(self.h, o.ctypes.data, float(step), m.ctypes.data,
+                                             float(energy), float(rest_energy), None,
+                                             ATX_STRICT if strict else 0), self.lib)
+        return m.reshape(self.nelem, 6, 6).transpose(0, 2, 1).copy()
 
     def check(self, energy: float, rest_energy: float) -> None:
         _check(self.lib.atx_lattice_check(self.h, float(energy), float(rest_energy)), self.lib)

@@ -70,7 +70,7 @@ class ELossMethod(Enum):
 
 
 # Tracking hooks: one turn in place (returns the refpt copies (6, N, R, 1)),
-# and one element.
+# one element, and the matrices of every element of a ring in one launch.
 def _lpass(ring, r_in, refpts=None, keep_lattice=False):
     refs = np.zeros(0, np.uint32) if refpts is None else refpts
     return tracking.internal_lpass(ring, r_in, 1, refpts=refs, keep_lattice=keep_lattice)
@@ -78,6 +78,10 @@ def _lpass(ring, r_in, refpts=None, keep_lattice=False):
 
 def _epass(elem, r_in, **kwargs):
     return tracking.internal_epass(elem, r_in, **kwargs)
+
+
+def _elements(ring, orbits, step, **kwargs):
+    return tracking.element_matrices(ring, orbits, step=step, **kwargs)
 
 
 def _refs(ring, refpts) -> np.ndarray:
@@ -438,3 +442,15 @@ def find_elem_m66(elem, orbit=None, **kwargs):
     batch, varied = _probe_batch(np.zeros(6) if orbit is None else orbit, steps)
     _epass(elem, batch, **kwargs)
     return _differences(batch, steps, varied)
+
+
+def element_matrices(ring, orbits=None, **kwargs):
+    """find_elem_m66 of every element of ``ring`` at once: the 6x6 matrix of
+    each element around the orbit at its entrance (``orbits`` (6, nelem),
+    e.g. the closed orbit at every element; default 0) by central
+    differences with step XYStep. pyat's _dmatr makes these calls element by
+    element (pyat/at/physics/radiation.py:78-90, matrix.py:171-202); here
+    they are one engine launch. Keywords: XYStep, energy, particle (default:
+    the ring's), strict. Returns (nelem, 6, 6)."""
+    xy = kwargs.pop("XYStep", DConstant.XYStep)
+    return _elements(ring, orbits, xy, **kwargs)

@@ -22,7 +22,8 @@ from .lattice import AtError, Element, Lattice, Particle
 __all__ = [
     "MPMode", "lattice_track", "element_track", "internal_lpass", "internal_epass",
     "lattice_pass", "element_pass", "gpu_info", "gpu_core_count", "gpupass", "atpass",
-    "get_uint32_index", "End", "diffusion_matrix", "diffusion_matrices", "patpass",
+    "get_uint32_index", "End", "diffusion_matrix", "diffusion_matrices", "element_matrices",
+    "patpass",
     "internal_plpass",
 ]
 
@@ -342,6 +343,34 @@ def diffusion_matrices(lattice, orbits, *, energy=None, particle=None,
         _Cache.device_lattice = engine.DeviceLattice(lattice)
         _Cache.lattice_id = id(lattice)
     return _Cache.device_lattice.diffusion_matrices(orbits, e, rest)
+
+
+def element_matrices(lattice, orbits=None, *, step: float, energy=None, particle=None,
+                     strict: bool = False, keep_lattice: bool = False) -> np.ndarray:
+    """6x6 matrix of every element of a lattice around its entrance orbit
+    (orbits (6, nelem), default 0) by central differences of ``step``, in one
+    launch; returns (nelem, 6, 6). Element by element this equals
+    find_elem_m66 (pyat/at/physics/matrix.py:171-202), which pyat's _dmatr
+    calls in a loop (physics/radiation.py:78-90). Energy and particle default
+    to the lattice's. The device lattice is shared with gpupass and reused
+    with keep_lattice."""
+    kw = {}
+    if energy is not None:
+        kw["energy"] = energy
+    if particle is not None:
+        kw["particle"] = particle
+    e, rest, _ = _energy_particle(lattice, kw)
+    elems = list(lattice)
+    if orbits is None:
+        orbits = np.zeros((6, len(elems)))
+    orbits = np.asfortranarray(orbits, dtype=np.float64)
+    if not (keep_lattice and _Cache.lattice_id == id(lattice)
+            and _Cache.device_lattice is not None):
+        _Cache.device_lattice = None
+        _Cache.lattice_id = None
+        _Cache.device_lattice = engine.DeviceLattice(elems)
+        _Cache.lattice_id = id(lattice)
+    return _Cache.device_lattice.element_matrices(orbits, step, e, rest, strict=strict)
 
 
 def gpu_info() -> list:

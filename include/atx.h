@@ -238,6 +238,23 @@ int atx_adjoint_batch(atx_lattice *lat, const atx_adjoint_item *items, int32_t n
 int atx_diffusion_matrices(atx_lattice *lat, const double *orbits, double *bdiff, double energy,
                            double rest_energy, void *stream, uint32_t flags);
 
+/* --- Element matrices ----------------------------------------------------
+ * Replaces find_elem_m66 (pyat/at/physics/matrix.py:171-202), which
+ * _dmatr calls once per element (pyat/at/physics/radiation.py:78-90), each
+ * call an at_elempass of 12 particles (pyat/at.c at_elempass): for every
+ * element of the lattice at once, its 6x6 matrix by central differences
+ * around the orbit at its entrance. Probe j+ / j- is the orbit with +step/2
+ * / -step/2 added to coordinate j (matrix.py:195-199), passed through that
+ * element alone (no loss check); m[i][j] = (probe j+ - probe j-)_i / step.
+ * One tracking launch over all elements (one instance each).
+ *   orbits  (6, nelem) F-order: orbit at the entrance of each element
+ *   m66     nelem blocks of 36 doubles, column-major 6x6 ([i][j] at i+6j)
+ *   step    the reference's XYStep (nonzero)
+ * Host arrays (synchronous) or, with ATX_DEVICE_POINTERS, device arrays
+ * (asynchronous on stream). ATX_STRICT: the strict build. */
+int atx_element_matrices(atx_lattice *lat, const double *orbits, double step, double *m66,
+                         double energy, double rest_energy, void *stream, uint32_t flags);
+
 /* Replace an attribute of element `index` (e.g. PolynomB of a magnet whose
  * strength is a differentiation parameter); the compiled lattice is
  * rebuilt on the next call. No counterpart in the reference, which

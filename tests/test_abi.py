@@ -193,3 +193,22 @@ def test_invalid_launch_shapes_are_value_errors():
             dl.set_launch_shape(*shape)
     for shape in ((64, 1, 1), (512, 2, 2), (0, 2, 0), (0, 0, 0)):
         dl.set_launch_shape(*shape)
+
+
+def test_element_matrices_argument_errors():
+    """atx_element_matrices rejects a NULL lattice, NULL arrays and a zero
+    step before any device work."""
+    lib = engine.load_library()
+    lat = engine.DeviceLattice([Drift("d", 1.0)])
+    o = np.zeros(6)
+    m = np.zeros(36)
+    assert lib.atx_element_matrices(None, o.ctypes.data, 1e-8, m.ctypes.data, 3e9, 0.0, None,
+                                    0) == engine.ATX_E_VALUE
+    assert lib.atx_element_matrices(lat.h, None, 1e-8, None, 3e9, 0.0, None,
+                                    0) == engine.ATX_E_VALUE
+    assert lib.atx_element_matrices(lat.h, o.ctypes.data, 0.0, m.ctypes.data, 3e9, 0.0, None,
+                                    0) == engine.ATX_E_VALUE
+    assert "step" in lib.atx_last_error().decode()
+    with pytest.raises(ValueError):
+        lat.element_matrices(np.zeros((6, 2)), 1e-8, 3e9, 0.0)
+
