@@ -120,15 +120,19 @@ def _roll(theta: float) -> np.ndarray:
 
 
 def error_realisation(ring: Lattice, seed: int, shift: float = 30e-6, roll: float = 100e-6,
-                      gradient: float = 1e-3, cut: float = 2.5) -> Lattice:
+                      gradient: float = 1e-3, cut: float = 2.5, settings=None) -> Lattice:
     """Copy of ``ring`` with seeded errors on every quadrupole and sextupole:
     transverse shifts (T1 = -d, T2 = +d), rolls about s (R1 = R(theta),
     R2 = R(-theta)) and relative strength errors on PolynomB[1]
     (quadrupoles) / PolynomB[2] (sextupoles), each Gaussian truncated at
-    ``cut`` sigma."""
+    ``cut`` sigma. The draws do not depend on the sizes, so scaling all three
+    scales one realisation. ``settings``: orbit-correction actuator settings
+    (one per ``correction_actuators`` entry) applied to the copy."""
     rng = np.random.default_rng(seed)
+    acts = correction_actuators(ring) if settings is not None else []
+    own = {k for k, _ in acts}
     elems = []
-    for e in ring:
+    for i, e in enumerate(ring):
         if isinstance(e, (Quadrupole, Sextupole)):
             e = e.deepcopy()
             dx, dy = _truncnorm(rng, shift, cut, 2)
@@ -141,10 +145,197 @@ def error_realisation(ring: Lattice, seed: int, shift: float = 30e-6, roll: floa
             pb = e.PolynomB.copy()
             pb[2 if isinstance(e, Sextupole) else 1] *= 1.0 + dk
             e.PolynomB = pb
+        elif i in own:
+            e = e.deepcopy()
         elems.append(e)
     lat = ring.copy()
     list.__init__(lat, elems)
+    if settings is not None:
+        if len(settings) != len(acts):
+            raise ValueError(f"{len(settings)} settings for {len(acts)} actuators")
+        for (k, plane), v in zip(acts, settings):
+            _set_kick(lat[k], plane, float(v))
     return lat
+
+
+# --- Orbit correction of the error realisations --------------------------
+# With 30 um rms shifts the large ring has a closed orbit of about 0.5-0.8 mm
+# rms already at a tenth of the errors, and none at their full size: the
+# realisations that are tracked for many turns are orbit-corrected first, as
+# error studies of such rings are. Actuators: the horizontal and vertical
+# kicks of the 3 correctors per cell and the dipole coils of the 6
+# sextupoles per cell (PolynomB[0], PolynomA[0]); the orbit is read at the
+# entrance of every quadrupole, sextupole and corrector (800 points). The
+# errors are ramped up from zero with the correction applied at each step,
+# (step halved when the closed orbit is lost), and the correction is the
+# Tikhonov-regularised least-squares solution of the ideal ring's response
+# matrix. The settings of 64 realisations are frozen in
+# data/large_ring_corrections.npz (written by tests/golden/make_corrections.py
+# on the engine).
+
+def correction_actuators(ring) -> list:
+    """(element index, plane) of every orbit-correction actuator, horizontal
+    kicks first (0: x, 1: y): correctors, then sextupole dipole coils."""
+    cors = [i for i, e in enumerate(ring) if isinstance(e, Corrector)]
+    sexts = [i for i, e in enumerate(ring) if isinstance(e, Sextupole)]
+    return [(k, p) for p in (0, 1) for k in cors + sexts]
+
+
+def correction_bpms(ring) -> list:
+    """Element indices where the orbit is read (their entrances)."""
+    return [i for i, e in enumerate(ring) if isinstance(e, (Quadrupole, Sextupole, Corrector))]
+
+
+def _kick(e, plane: int) -> float:
+    """Kick angle of an actuator: KickAngle of a corrector; -L B0 (x) or
+    +L A0 (y) of a sextupole's dipole coil (px -= L B0, py += L A0)."""
+    if isinstance(e, Corrector):
+        return float(e.KickAngle[plane])
+    return float(-e.PolynomB[0] * e.Length if plane == 0 else e.PolynomA[0] * e.Length)
+
+
+def _set_kick(e, plane: int, value: float) -> None:
+    if isinstance(e, Corrector):
+        ka = np.array(e.KickAngle, dtype=float)
+        ka[plane] = value
+        e.KickAngle = ka
+    elif plane == 0:
+        pb = np.array(e.PolynomB, dtype=float)
+        pb[0] = -value / e.Length
+        e.PolynomB = pb
+    else:
+        pa = np.array(e.PolynomA, dtype=float)
+        pa[0] = value / e.Length
+        e.PolynomA = pa
+
+
+def _bpm_orbit(ring, bpms, guess):
+    """Closed orbit (6,) and (x..., y...) at the bpms, or (None, None) when
+    the 4-D closed-orbit search does not converge."""
+    import warnings
+
+    from . import physics
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        try:
+            o, b = physics.find_orbit4(ring, 0.0, bpms, guess=guess)
+        except Exception:  # no closed orbit (non-convergence warning or a lost probe)
+            return None, None
+    if not np.all(np.isfinite(b)):
+        return None, None
+    return o, np.concatenate([b[:, 0], b[:, 2]])
+
+
+def orbit_response(ring, kick: float = 1e-6) -> np.ndarray:
+    """Response of the orbit at the bpms to each actuator, (2 nbpm,
+    nactuator), by one-sided differences of ``kick`` on ``ring`` (4-D)."""
+    bpms, acts = correction_bpms(ring), correction_actuators(ring)
+    _, b0 = _bpm_orbit(ring, bpms, None)
+    if b0 is None:
+        raise ValueError("the ring has no closed orbit")
+    elems = list(ring)
+    out = np.empty((b0.size, len(acts)))
+    for j, (k, plane) in enumerate(acts):
+        e0 = elems[k]
+        e1 = e0.deepcopy()
+        _set_kick(e1, plane, _kick(e0, plane) + kick)
+        ring[k] = e1
+        try:
+            _, b = _bpm_orbit(ring, bpms, None)
+        finally:
+            ring[k] = e0
+        if b is None:
+            raise ValueError(f"no closed orbit with actuator {j} kicked")
+        out[:, j] = (b - b0) / kick
+    return out
+
+
+def correct_realisation(base, seed: int, response: np.ndarray, *, shift: float = 30e-6,
+                        roll: float = 100e-6, gradient: float = 1e-3, reg: float = 1e-3,
+                        step0: float = 0.03, min_step: float = 0.005, final: int = 8,
+                        max_rms: float = 100e-6):
+    """Actuator settings that correct the closed orbit of realisation
+    ``seed`` of ``base`` (a 4-D ring: no radiation, no cavity), or None when
+    the ramp fails. Each step raises the error scale by ``step`` (starting at
+    ``step0``, x1.5 after a success up to 0.25, halved after a failure down
+    to ``min_step``) and applies two corrections; at full size corrections
+    continue while they reduce the orbit, at most ``final``. A realisation
+    whose corrected orbit stays above ``max_rms`` rms in either plane counts
+    as failed. Returns (settings, (rms x, rms y) of the corrected orbit at
+    the bpms)."""
+    bpms, acts = correction_bpms(base), correction_actuators(base)
+    u, sv, vt = np.linalg.svd(response, full_matrices=False)
+    gain = vt.T * (sv / (sv * sv + (sv[0] * reg) ** 2))
+
+    def corrected(scale, settings, guess, iterations):
+        ring = error_realisation(base, seed, shift * scale, roll * scale, gradient * scale,
+                                 settings=settings)
+        for _ in range(iterations):
+            o, b = _bpm_orbit(ring, bpms, guess)
+            if o is None:
+                return None, None, None
+            guess = o
+            settings = settings - gain @ (u.T @ b)
+            for (k, plane), v in zip(acts, settings):
+                _set_kick(ring[k], plane, float(v))
+        return settings, guess, ring
+
+    settings = np.array([_kick(base[k], p) for k, p in acts])
+    guess, done, step = np.zeros(6), 0.0, step0
+    while done < 1.0:
+        scale = min(1.0, done + step)
+        s2, g2, _ = corrected(scale, settings, guess, 2)
+        if s2 is None:
+            step *= 0.5
+            if step < min_step:
+                return None
+            continue
+        settings, guess, done = s2, g2, scale
+        step = min(0.25, 1.5 * step)
+    n = len(bpms)
+
+    def rms(settings, guess):
+        ring = error_realisation(base, seed, shift, roll, gradient, settings=settings)
+        o, b = _bpm_orbit(ring, bpms, guess)
+        return (None, None) if o is None else ((float(np.std(b[:n])), float(np.std(b[n:]))), o)
+
+    best, guess = rms(settings, guess)
+    if best is None:
+        return None
+    for _ in range(final):
+        s2, g2, _ = corrected(1.0, settings, guess, 1)
+        if s2 is None:
+            break
+        r2, g3 = rms(s2, g2)
+        if r2 is None or max(r2) >= max(best):
+            break
+        settings, guess, best = s2, g3, r2
+    if max(best) > max_rms:
+        return None
+    return settings, best
+
+
+_CORRECTIONS = os.path.join(_DATA, "large_ring_corrections.npz")
+
+
+def frozen_corrections() -> dict:
+    """The frozen orbit corrections of the large ring's error realisations:
+    {"seeds": (n,), "settings": (n, nactuator), "orbit_rms": (n, 2), and the
+    error sizes "shift", "roll", "gradient"}."""
+    with np.load(_CORRECTIONS, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def corrected_realisations(ring: Lattice, n: int) -> list:
+    """The first ``n`` frozen orbit-corrected error realisations of ``ring``
+    (the large ring, with or without radiation and cavity)."""
+    c = frozen_corrections()
+    if n > len(c["seeds"]):
+        raise ValueError(f"{len(c['seeds'])} corrected realisations are frozen, {n} asked")
+    return [error_realisation(ring, int(c["seeds"][i]), float(c["shift"]), float(c["roll"]),
+                              float(c["gradient"]), settings=c["settings"][i])
+            for i in range(n)]
 
 
 def da_polar_grid(n: int, xmax: float = 15e-3, ymax: float = 6e-3,
@@ -172,25 +363,20 @@ def da_polar_grid(n: int, xmax: float = 15e-3, ymax: float = 6e-3,
     return np.asfortranarray(np.concatenate(cols, axis=1))
 
 
-# Transverse shift of quadrupoles and sextupoles in the error realisations
-# that are tracked for many turns: the uncorrected large ring has no closed
-# orbit with 30 um rms shifts (every particle is lost within 3 turns), so
-# these workloads use 3 um, the residual level after orbit correction; rolls
-# (100 urad) and gradient errors (1e-3) keep their full size.
-RESIDUAL_SHIFT = 3e-6
-
-
 def batched_da_workload(ninst: int = 64, seed: int = 7, nmin: int = 1000, nmax: int = 40000,
-                        ncells: int | None = None, shift: float = RESIDUAL_SHIFT):
-    """Error-seeded copies of the large ring with DA grids of U[nmin, nmax]
-    particles each: a list of (lattice, particles)."""
+                        shift: float | None = None):
+    """Error realisations of the large ring with DA grids of U[nmin, nmax]
+    particles each: a list of (lattice, particles). Default: the frozen
+    orbit-corrected realisations with 30 um shifts (corrected_realisations);
+    ``shift``: uncorrected realisations with that rms shift instead (seeds
+    1000 seed + i)."""
     rng = np.random.default_rng(seed)
-    base = large_ring(ncells=ncells)
-    out = []
-    for i in range(ninst):
-        n = int(rng.integers(nmin, nmax + 1))
-        out.append((error_realisation(base, seed * 1000 + i, shift=shift), da_polar_grid(n)))
-    return out
+    base = large_ring()
+    if shift is None:
+        lats = corrected_realisations(base, ninst)
+    else:
+        lats = [error_realisation(base, seed * 1000 + i, shift=shift) for i in range(ninst)]
+    return [(lat, da_polar_grid(int(rng.integers(nmin, nmax + 1)))) for lat in lats]
 
 
 def fodo_ring(ncells: int = 32, energy: float = 3.0e9) -> Lattice:

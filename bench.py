@@ -122,8 +122,9 @@ def bench_batch(args) -> dict:
     tracked for ``--batch-turns`` turns (1,024 by default) in ONE launch
     (launch shape from the autotuner), against the same launch without
     repacking and, with ``--per-instance``, the instances tracked one launch
-    each. ``--shift`` sets the rms quadrupole/sextupole shift (3 um residual
-    by default; 30 um is the uncorrected size)."""
+    each. The realisations are the frozen orbit-corrected ones with 30 um rms
+    shifts; ``--shift`` takes uncorrected realisations with that rms shift
+    instead."""
     import numpy as np
 
     from at_amd import device, engine
@@ -202,7 +203,9 @@ def bench_batch(args) -> dict:
         "data": "synthetic",
         "config": {"workload": f"{len(dls)} error realisations of large_ring ({len(lats[0])} "
                                f"elements; rolls 100 urad, gradients 1e-3, shifts "
-                               f"{args.shift * 1e6:g} um rms), DA polar grids of U[1000, 40000] "
+                               + ("30 um rms, orbit-corrected" if args.shift is None else
+                                  f"{args.shift * 1e6:g} um rms, uncorrected")
+                               + f"), DA polar grids of U[1000, 40000] "
                                f"particles ({ntot} total), {T} turns, one launch"
                                + ("" if args.no_repack else
                                   " (turn blocks with survivors repacked, one graph)"),
@@ -220,10 +223,10 @@ def bench_batch(args) -> dict:
 
 def _serving_setup(n_instances: int):
     from at_amd import engine
-    from at_amd.workloads import RESIDUAL_SHIFT, error_realisation, large_ring
+    from at_amd.workloads import corrected_realisations, large_ring
 
     base = large_ring()
-    lats = [error_realisation(base, 1000 + i, shift=RESIDUAL_SHIFT) for i in range(n_instances)]
+    lats = corrected_realisations(base, n_instances)
     dls = [engine.DeviceLattice(l) for l in lats]
     for dl in dls:
         dl.check(base.energy, base.particle.rest_energy)
@@ -296,8 +299,9 @@ def serving_sweep(args) -> dict:
         "trace": {"requests": args.serve_requests, "seed": args.serve_seed,
                   "instances": len(dls), "particles": "log-uniform [2e3, 2e5]",
                   "turns": list(turns), "kinds": "half DA grids, half Gaussian beams",
-                  "lattices": f"error realisations of large_ring ({len(base)} elements), "
-                              f"resident on the device", "turn_block": args.turn_block},
+                  "lattices": f"orbit-corrected error realisations (30 um shifts) of "
+                              f"large_ring ({len(base)} elements), resident on the device",
+                  "turn_block": args.turn_block},
         "points": points,
         "at_equal_p99": {"p99_bound_s": bound,
                          "batched_passes_per_s": best["batched"],
@@ -335,8 +339,9 @@ def bench_serve(args) -> dict:
         "data": "synthetic",
         "config": {"workload": f"{len(trace)} requests, Poisson {args.rate}/s, "
                                f"{len(dls)} error realisations of large_ring ({len(base)} "
-                               f"elements; rolls 100 urad, gradients 1e-3, residual shifts "
-                               f"3 um), N log-uniform [2e3, 2e5], turns in {{256, 1024, 4096}}",
+                               f"elements; rolls 100 urad, gradients 1e-3, shifts 30 um rms, "
+                               f"orbit-corrected), N log-uniform [2e3, 2e5], turns in "
+                               f"{{256, 1024, 4096}}",
                    "turn_block": args.turn_block,
                    "requests": [[q.npart, q.nturns, q.kind] for q in trace]},
         "batched": b, "one_launch_per_request": p,
@@ -530,8 +535,9 @@ def main() -> int:
                          "bandwidth of the copies)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-turns", type=int, default=1024)
-    ap.add_argument("--shift", type=float, default=3e-6,
-                    help="batch: rms quadrupole/sextupole shift of the error realisations [m]")
+    ap.add_argument("--shift", type=float, default=None,
+                    help="batch: uncorrected error realisations with this rms quadrupole/"
+                         "sextupole shift [m] (default: the orbit-corrected 30 um ones)")
     ap.add_argument("--no-repack", action="store_true",
                     help="batch/serve: one plain launch instead of repacked turn blocks")
     ap.add_argument("--requests", type=int, default=12)

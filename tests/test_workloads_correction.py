@@ -1,0 +1,84 @@
+"""CPU: orbit correction of the error realisations (at_amd/workloads.py) —
+the frozen settings file, the actuator conventions, the scaling of one
+realisation, and the ramped correction itself on a 4-cell ring with the
+oracle doing the tracking (tests/oracle_hooks.py)."""
+
+import numpy as np
+import pytest
+
+from at_amd import physics, workloads
+from at_amd.lattice import Corrector, Sextupole
+from oracle import pyoracle as po
+
+import oracle_hooks
+
+
+def test_frozen_corrections_file():
+    c = workloads.frozen_corrections()
+    ring = workloads.large_ring()
+    n = len(c["seeds"])
+    assert n >= 64
+    assert c["settings"].shape == (n, len(workloads.correction_actuators(ring)))
+    assert np.all(np.diff(c["seeds"]) > 0)
+    assert not set(c["seeds"]) & set(c.get("skipped", np.zeros(0, np.int64)))
+    assert float(c["shift"]) == 30e-6 and float(c["roll"]) == 100e-6
+    assert np.all(c["orbit_rms"] <= 100e-6)
+    assert np.all(np.isfinite(c["settings"])) and np.abs(c["settings"]).max() < 2e-3
+
+
+def test_actuator_settings_round_trip():
+    ring = workloads.large_ring()
+    acts = workloads.correction_actuators(ring)
+    ncor = sum(isinstance(e, Corrector) for e in ring)
+    nsext = sum(isinstance(e, Sextupole) for e in ring)
+    assert len(acts) == 2 * (ncor + nsext)
+    st = np.linspace(-1e-4, 1e-4, len(acts))
+    lat = workloads.error_realisation(ring, 3, settings=st)
+    got = np.array([workloads._kick(lat[k], p) for k, p in acts])
+    np.testing.assert_allclose(got, st, rtol=1e-15, atol=0)
+    # the base ring's actuators are untouched (the realisation owns copies)
+    assert all(workloads._kick(ring[k], p) != v for (k, p), v in zip(acts, st) if v != 0)
+    # a sextupole coil: px -= L B0, py += L A0 (a corrector's KickAngle sign)
+    k = next(k for k, p in acts if isinstance(ring[k], Sextupole) and p == 0)
+    e = lat[k]
+    assert np.sign(e.PolynomB[0]) == -np.sign(workloads._kick(e, 0))
+
+
+def test_error_sizes_scale_one_realisation():
+    ring = workloads.large_ring(radiation=False, cavity=False)
+    a = workloads.error_realisation(ring, 11, 30e-6, 100e-6, 1e-3)
+    b = workloads.error_realisation(ring, 11, 3e-6, 10e-6, 1e-4)
+    for ea, eb, e0 in zip(a, b, ring):
+        if hasattr(ea, "T1") and ea is not e0:
+            np.testing.assert_allclose(eb.T1, 0.1 * ea.T1, rtol=1e-14, atol=0)
+
+
+@pytest.mark.skipif(not po.available("oracle"), reason="oracle not built")
+def test_ramped_correction_on_four_cells(monkeypatch):
+    """On a 4-cell ring the ramp reaches the full error size, and the
+    corrected orbit is far smaller than the uncorrected one, which (linear in
+    small errors) is 10 times the orbit at a tenth of the errors."""
+    oracle_hooks.install(monkeypatch)
+    ring = workloads.large_ring(radiation=False, cavity=False, ncells=4)
+    bpms = workloads.correction_bpms(ring)
+    resp = workloads.orbit_response(ring)
+    assert resp.shape == (2 * len(bpms), len(workloads.correction_actuators(ring)))
+    for seed in range(40, 44):
+        raw = workloads.error_realisation(ring, seed, 3e-6, 10e-6, 1e-4)
+        _, b = workloads._bpm_orbit(raw, bpms, None)
+        if b is None:
+            continue
+        res = workloads.correct_realisation(ring, seed, resp)
+        if res is None:
+            continue
+        settings, (rx, ry) = res
+        n = len(bpms)
+        assert rx < 0.2 * 10 * np.std(b[:n])
+        assert ry < 0.2 * 10 * np.std(b[n:])
+        # the settings applied to a fresh copy give that orbit
+        lat = workloads.error_realisation(ring, seed, settings=settings)
+        o, b2 = workloads._bpm_orbit(lat, bpms, None)
+        assert o is not None
+        np.testing.assert_allclose(np.std(b2[:n]), rx, rtol=1e-3)
+        return
+    pytest.fail("no seed of 40-43 was corrected")
