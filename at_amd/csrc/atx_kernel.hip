@@ -1419,11 +1419,16 @@ __device__ __forceinline__ void element_misc(R6& r, cDevElem& h, cdouble* pool, 
 
 // The P particles of a lane through one element: drifts and thick
 // multipoles step all P together, the other kinds run per particle.
+// A drift's length, kept in the header words its kind does not use.
+__device__ __forceinline__ double drift_length(cDevElem& h) {
+  const uint64_t bits = ((uint64_t)(uint32_t)h.fringe << 32) | (uint32_t)h.edge;
+  return __builtin_bit_cast(double, bits);
+}
+
 template <bool EXT, int P, int U>
 __device__ __forceinline__ void element_lane(R6* r, cDevElem& h, cdouble* pool, int nturn) {
-  if (h.kind == K_DRIFT) {  // DriftPass.c:16-48; half the elements of a ring, tested first
-    const uint64_t bits = ((uint64_t)(uint32_t)h.fringe << 32) | (uint32_t)h.edge;
-    const double len = __builtin_bit_cast(double, bits);
+  if (h.kind == K_DRIFT) {  // DriftPass.c:16-48 with misalignment or apertures
+    const double len = drift_length(h);
 #pragma unroll
     for (int q = 0; q < P; q++) {
       entry_xform(r[q], h, pool);
@@ -1672,7 +1677,16 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
           if (has[q] && sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p[q], 1, r[q]);
       }
       cDevElem& h = elems[e];
-      if (P == 1) {
+      if (h.kind == K_DRIFT && h.flags == 0) {
+        // A drift without misalignment or apertures (DriftPass.c:16-48), half
+        // the elements of a ring: tested first, from the header's first two
+        // words, and stepped without the element dispatch. Every element
+        // skips a particle whose x is NaN (at.c:142-161).
+        const double len = drift_length(h);
+#pragma unroll
+        for (int q = 0; q < P; q++)
+          if (!__builtin_isnan(r[q].x)) drift6(r[q], len);
+      } else if (P == 1) {
         if (!__builtin_isnan(r[0].x)) element_lane<EXT, 1, U>(r, h, pool, nturn);
       } else {
         // A lane steps its particles together. Every element skips a

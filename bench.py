@@ -192,6 +192,25 @@ def bench_batch(args) -> dict:
         stream.synchronize()
         seq_ms = s0.elapsed_ms(s1)
         same = all(_same(a, b) for a, b in zip(parts, parts2))
+    # every shape the autotuner scored, run as the whole launch
+    tune_full = None
+    if args.tune_check:
+        tune_full = {}
+        for name in sorted(tune["ms"]):
+            b, rest_ = name[1:].split("p")
+            p, u = rest_.split("u")
+            parts4 = [x.clone(stream) for x in base]
+            (t0_, t1_), = _pairs(1)
+            stream.synchronize()
+            t0_.record(stream)
+            track_batch(dls, parts4, T, energy=energy, rest_energy=rest, losses=True,
+                        repack=not args.no_repack, stream=stream, block_size=int(b), ppt=int(p),
+                        unroll=int(u))
+            t1_.record(stream)
+            stream.synchronize()
+            tune_full[name] = t0_.elapsed_ms(t1_)
+            print(f"batch: shape {name}: {tune_full[name]:.1f} ms (model {tune['ms'][name]:.0f})",
+                  file=sys.stderr, flush=True)
     achieved = flops / (ms * 1e-3) / 1e12
     ntot = sum(int(b.shape[0]) for b in base)
     return {
@@ -218,6 +237,12 @@ def bench_batch(args) -> dict:
         "one_launch_per_instance_ms": seq_ms,
         "speedup_vs_one_launch_per_instance": (seq_ms / ms) if seq_ms else None,
         "identical_to_one_launch_per_instance": same,
+        "autotune_check": None if tune_full is None else {
+            "full_launch_ms": tune_full,
+            "chosen": f"b{shape['block_size']}p{shape['ppt']}u{shape['unroll']}",
+            "best": min(tune_full, key=tune_full.get),
+            "chosen_over_best": tune_full[f"b{shape['block_size']}p{shape['ppt']}u{shape['unroll']}"]
+            / min(tune_full.values())},
     }
 
 
@@ -555,6 +580,8 @@ def main() -> int:
     ap.add_argument("--adj-particles", type=int, default=4096)
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--tune-check", action="store_true",
+                    help="batch: also run every shape the autotuner scored as the whole launch")
     ap.add_argument("--per-instance", action="store_true",
                     help="batch: also time one launch per instance (about 6 minutes at 1,024 "
                          "turns)")
