@@ -84,6 +84,65 @@ def _elements(ring, orbits, step, **kwargs):
     return tracking.element_matrices(ring, orbits, step=step, **kwargs)
 
 
+class _Rows:
+    """Rows [start, start + n) of a device buffer (N, 6), as track_batch takes
+    particles (shape, dtype, ptr)."""
+
+    def __init__(self, buf, start: int, n: int):
+        self.shape = (n, 6)
+        self.dtype = np.dtype(np.float64)
+        self.ptr = buf.ptr + 48 * start
+
+
+class _DeviceRings:
+    """One-turn passes of the probes of many rings, one batched launch
+    (atx_track_batch) per call; the device lattices are built once."""
+
+    def __init__(self, rings, strict: bool = False):
+        from . import engine
+
+        kw = {}
+        self.energy, self.rest, _ = tracking._energy_particle(rings[0], kw)
+        for r in rings[1:]:
+            e, rest, _ = tracking._energy_particle(r, {})
+            if (e, rest) != (self.energy, self.rest):
+                raise AtError("the rings of a batch must share energy and particle")
+        self.strict = strict
+        self.dls = [engine.DeviceLattice(list(r)) for r in rings]
+
+    def turn(self, idx, probes) -> None:
+        """Probes (6, k) of the rings ``idx`` through one turn, in place."""
+        from . import device
+        from .batch import track_batch
+
+        k = probes[0].shape[1]
+        host = np.concatenate([p.T for p in probes])
+        buf = device.DeviceArray.from_host(host)
+        try:
+            track_batch([self.dls[i] for i in idx], [_Rows(buf, j * k, k) for j in range(len(idx))],
+                        1, energy=self.energy, rest_energy=self.rest, losses=False,
+                        strict=self.strict)
+            out = buf.to_host()
+        finally:
+            buf.close()
+        for j, p in enumerate(probes):
+            p[:] = out[j * k:(j + 1) * k].T
+
+    def along(self, i, orbit, refs) -> np.ndarray:
+        r = np.asfortranarray(np.array(orbit, dtype=float).reshape(6, 1))
+        rout = self.dls[i].track(r, 1, refs, energy=self.energy, rest_energy=self.rest,
+                                 strict=self.strict)
+        return rout[:, 0, :, 0].T.copy()
+
+    def close(self) -> None:
+        for dl in self.dls:
+            dl.close()
+
+
+# Tracking hook of the batched solvers: a factory of _DeviceRings-like objects.
+_rings = _DeviceRings
+
+
 def _refs(ring, refpts) -> np.ndarray:
     return tracking.get_uint32_index(ring, refpts)
 
@@ -454,3 +513,48 @@ def element_matrices(ring, orbits=None, **kwargs):
     the ring's), strict. Returns (nelem, 6, 6)."""
     xy = kwargs.pop("XYStep", DConstant.XYStep)
     return _elements(ring, orbits, xy, **kwargs)
+
+
+def find_orbit4_batch(rings, dp=0.0, refpts=None, *, guesses=None, strict=False, **kwargs):
+    """find_orbit4 at momentum deviation ``dp`` for many rings at once (the
+    error realisations of a lattice, say): every Newton iteration is one
+    batched launch of the probes of all rings not yet converged, each ring
+    with its own Newton state, steps and stopping rule exactly as
+    find_orbit4 (orbit.py:16-72). The rings must share energy and particle.
+    ``guesses``: starting points (R, 6). Returns (orbits (R, 6), list of the
+    orbits at ``refpts`` (nref, 6) per ring, or None without refpts); warns
+    for the rings that did not converge (their last iterate is returned)."""
+    conv, maxit, xy, _, _ = _options(kwargs, "orbit_dp")
+    nr = len(rings)
+    xs = np.zeros((nr, 6)) if guesses is None else np.array(guesses, dtype=float).reshape(nr, 6)
+    xs[:, 4] = 0.0 if dp is None else dp
+    t = np.arange(4)
+    steps = np.zeros(6)
+    steps[t] = xy
+    tracker = _rings(rings, strict=strict)
+    try:
+        active = list(range(nr))
+        for _ in range(maxit):
+            if not active:
+                break
+            batches = [_probe_batch(xs[i], steps) for i in active]
+            probes = [b for b, _ in batches]
+            tracker.turn(active, probes)
+            still = []
+            for i, (probe, varied) in zip(active, batches):
+                image = probe[:, 0]
+                jac = _differences(probe, steps, varied)
+                update = np.linalg.solve(jac[t] - np.eye(4), image[t] - xs[i, t])
+                xs[i, t] -= update
+                if not np.linalg.norm(update) <= conv:
+                    still.append(i)
+            active = still
+        if active:
+            warnings.warn(AtWarning(f"Maximum number of iterations reached for rings {active}. "
+                                    "Possible non-convergence"), stacklevel=2)
+        along = None
+        if refpts is not None:
+            along = [tracker.along(i, xs[i], _refs(rings[i], refpts)) for i in range(nr)]
+    finally:
+        tracker.close()
+    return xs, along

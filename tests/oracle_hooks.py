@@ -35,8 +35,32 @@ def oracle_project(ring, parts, obs, **kwargs):
     po.atpass(ring, parts, 1, start_elem=int(obs), energy=e, particle=part)
 
 
+class OracleRings:
+    """The batched solvers' tracking hook on the oracle: each ring's probes
+    through one turn, ring by ring."""
+
+    def __init__(self, rings, strict=False):
+        self.rings = rings
+
+    def turn(self, idx, probes):
+        for i, p in zip(idx, probes):
+            e, part = _ep(self.rings[i])
+            po.atpass(self.rings[i], p, 1, energy=e, particle=part)
+
+    def along(self, i, orbit, refs):
+        e, part = _ep(self.rings[i])
+        r = np.asfortranarray(np.array(orbit, dtype=float).reshape(6, 1))
+        rout = po.atpass(self.rings[i], r, 1, refpts=np.asarray(refs, np.uint32), energy=e,
+                         particle=part)
+        return np.asarray(rout)[:, 0, :, 0].T.copy()
+
+    def close(self):
+        pass
+
+
 def install(monkeypatch):
     monkeypatch.setattr(physics, "_lpass", oracle_lpass)
     monkeypatch.setattr(physics, "_epass", oracle_epass)
+    monkeypatch.setattr(physics, "_rings", OracleRings)
     monkeypatch.setattr(acceptance, "_survived", oracle_survived)
     monkeypatch.setattr(acceptance, "_project", oracle_project)

@@ -134,3 +134,37 @@ def test_recursive_with_no_survivor_warns(hooks, hmba):
                                            bounds=[[0.5, 1.0], [np.pi, 0]], nturns=4)
     assert surv.shape[1] == 0
     np.testing.assert_array_equal(bnd, np.zeros((2, 1)))
+
+
+def _realisations(ring, n, shift=2e-6):
+    """Small seeded misalignments of hmba's quadrupoles (T1/T2 shifts)."""
+    rng = np.random.default_rng(5)
+    out = []
+    for _ in range(n):
+        elems = []
+        for e in ring:
+            if e.PassMethod.startswith("StrMPole") and len(e.PolynomB) > 1 and e.PolynomB[1] != 0:
+                e = e.deepcopy()
+                d = rng.normal(0.0, shift, 2)
+                e.T1 = np.array([-d[0], 0.0, -d[1], 0.0, 0.0, 0.0])
+                e.T2 = -e.T1
+            elems.append(e)
+        r = ring.copy()
+        list.__init__(r, elems)
+        out.append(r)
+    return out
+
+
+def test_find_orbit4_batch_equals_one_by_one(hooks, hmba):
+    """Each ring of the batch gets find_orbit4's orbit and refpt orbits
+    (the same Newton iterates, so equal to the last bit)."""
+    rings = _realisations(hmba.radiation_off(copy=True), 4)
+    orbits, along = physics.find_orbit4_batch(rings, 1e-4, [0, 20, 60])
+    for i, r in enumerate(rings):
+        o, pts = physics.find_orbit4(r, 1e-4, [0, 20, 60])
+        np.testing.assert_array_equal(orbits[i], o)
+        np.testing.assert_array_equal(along[i], pts)
+    assert np.abs(orbits[:, 0]).max() > 1e-7  # the shifts move the orbit
+    orbits2, none = physics.find_orbit4_batch(rings, 1e-4, guesses=orbits)
+    assert none is None
+    np.testing.assert_allclose(orbits2, orbits, rtol=0, atol=1e-12)
