@@ -118,7 +118,10 @@ def _layout(planes, npoints, amplitudes, mode, bounds=None, shift_zero=1.0e-6) -
 # --- tracking hooks (the engine) ------------------------------------------------
 
 def _survived(parts, ring, nturns, **kwargs):
-    """Survival of each particle after ``nturns`` (one launch, in place)."""
+    """Survival of each particle after ``nturns`` (one call, in place; a
+    grid loses most of its points in the first turns, so the survivors are
+    repacked between turn blocks)."""
+    kwargs.setdefault("repack", True)
     _, _, td = tracking.lattice_track(ring, parts, nturns=nturns, losses=True, refpts=None,
                                       in_place=True, **kwargs)
     return np.invert(td["loss_map"].islost)

@@ -919,6 +919,8 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
       const int hi = std::min(bounds[k + 1], (int)insts[i].nturns);
       in.nturns = std::max(0, hi - lo);
       in.turn0 = insts[i].turn0 + lo;
+      // a partial first turn (start_elem) is the first block's only
+      if (k > 0) in.start_elem = 0;
       if (k > 0) {
         in.losses = insts[i].losses ? 2 : 0;
         in.idx = (const int32_t*)(d + idx_off) + ioff;
@@ -1017,7 +1019,7 @@ int single_instance(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstr
 int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride, uint32_t np,
                  int nturns, const uint32_t* refpts, uint32_t nref, int turn0, int start, int end,
                  double* d_rout, uint8_t* d_lf, uint32_t* d_lt, uint32_t* d_le, double* d_lc,
-                 int losses, bool elempass, hipStream_t stream, bool strict) {
+                 int losses, bool elempass, hipStream_t stream, bool strict, bool repack = false) {
   const int nelem = (int)lat->elems.size();
   std::vector<std::vector<int32_t>> slots(1, std::vector<int32_t>(2 * (size_t)(nelem + 1)));
   std::vector<Instance> insts(1);
@@ -1037,6 +1039,10 @@ int track_device(atx_lattice* lat, double* d_r, int64_t pstride, int64_t cstride
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
       (int64_t)np >= (int64_t)2 * 512 * shape.ppt * ncu)
     shape.block = 512;
+  // survivors repacked between turn blocks: only without refpts (lost
+  // particles still fill their copies, at.c:622-626)
+  if (repack && nref == 0 && nturns > 1)
+    return launch_repacked(insts, {&slots[0]}, shape, stream, strict, lat->has_ext);
   return launch_instances(insts, {&slots[0]}, shape, stream, strict, lat->has_ext);
 }
 
@@ -1237,7 +1243,7 @@ int atx_track(atx_lattice* lat, double* r_in, uint32_t np, int32_t nturns, const
     const bool soa = (flags & ATX_SOA) != 0;
     rc = track_device(lat, r_in, soa ? 1 : 6, soa ? (int64_t)np : 1, np, nturns, refpts, nref, turn0,
                       start, end, r_out, lost_flag, lost_turn, lost_elem, lost_coord, losses, false,
-                      stream, strict);
+                      stream, strict, (flags & ATX_REPACK) != 0);
     if (rc) return rc;
   } else {
     // host buffers: stage through device memory
@@ -1278,7 +1284,7 @@ int atx_track(atx_lattice* lat, double* r_in, uint32_t np, int32_t nturns, const
       HIPCHK2(hipMalloc(&d_le, np * sizeof(uint32_t)));
     }
     rc = track_device(lat, d_r, 6, 1, np, nturns, refpts, nref, turn0, start, end, d_out, d_lf, d_lt,
-                      d_le, d_lc, losses, false, stream, strict);
+                      d_le, d_lc, losses, false, stream, strict, (flags & ATX_REPACK) != 0);
     if (rc) {
       cleanup();
       return rc;

@@ -359,9 +359,9 @@ class DeviceLattice:
     def track(self, r_in: np.ndarray, nturns: int, refpts=None, turn: int = 0,
               energy: float = 0.0, rest_energy: float = 0.0, charge: float = -1.0,
               losses: bool = False, start_elem: int = 0, end_elem: int = 0,
-              keep_counter: bool = False, strict: bool = False):
+              keep_counter: bool = False, strict: bool = False, repack: bool = False):
         """Host-array tracking with the at.c contract; returns rout or
-        (rout, loss dict)."""
+        (rout, loss dict). ``repack``: as track_device."""
         if r_in.ndim not in (1, 2) or r_in.shape[0] != 6:
             raise ValueError("rin is not 6D")
         if r_in.dtype != np.float64:
@@ -371,7 +371,8 @@ class DeviceLattice:
         npart = r_in.size // 6
         refs = np.zeros(0, np.uint32) if refpts is None else np.ascontiguousarray(refpts, np.uint32)
         rout = np.full((6, npart, refs.size, max(nturns, 0)), np.nan, order="F")
-        flags = (ATX_STRICT if strict else 0) | (ATX_KEEP_COUNTER if keep_counter else 0)
+        flags = ((ATX_STRICT if strict else 0) | (ATX_KEEP_COUNTER if keep_counter else 0)
+                 | (ATX_REPACK if repack else 0))
         lf = lt = le = lc = None
         if losses:
             lf = np.zeros(npart, np.uint8)
@@ -393,11 +394,14 @@ class DeviceLattice:
                      rout_ptr: int | None = None, turn: int = 0, energy: float = 0.0,
                      rest_energy: float = 0.0, losses: bool = False, loss_ptrs=None,
                      stream: int | None = None, strict: bool = False, soa: bool = False,
-                     start_elem: int = 0, end_elem: int = 0) -> None:
+                     start_elem: int = 0, end_elem: int = 0, repack: bool = False) -> None:
         """Asynchronous tracking of device-resident particles (raw device
-        pointers, e.g. torch ``tensor.data_ptr()``) on a HIP stream."""
+        pointers, e.g. torch ``tensor.data_ptr()``) on a HIP stream.
+        ``repack``: turn blocks with the survivors packed between them
+        (ATX_REPACK; without refpts; identical results)."""
         refs = np.zeros(0, np.uint32) if refpts is None else np.ascontiguousarray(refpts, np.uint32)
-        flags = ATX_DEVICE_POINTERS | (ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0)
+        flags = (ATX_DEVICE_POINTERS | (ATX_STRICT if strict else 0) | (ATX_SOA if soa else 0)
+                 | (ATX_REPACK if repack else 0))
         lf, lt, le, lc = loss_ptrs if loss_ptrs is not None else (None, None, None, None)
         rc = self.lib.atx_track(self.h, r_ptr, int(npart), int(nturns),
                                 refs.ctypes.data_as(_u32p), refs.size, int(turn), float(energy),

@@ -580,6 +580,8 @@ def main() -> int:
     ap.add_argument("--adj-particles", type=int, default=4096)
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--repack", choices=("on", "off"), default="off",
+                    help="track: repacked turn blocks (ATX_REPACK; gpupass's default is off)")
     ap.add_argument("--tune-check", action="store_true",
                     help="batch: also run every shape the autotuner scored as the whole launch")
     ap.add_argument("--per-instance", action="store_true",
@@ -641,11 +643,13 @@ def main() -> int:
     K, W = args.steps, args.warmup
     loss_sets = [LossMap.empty(n) for _ in range(W + K)]
 
+    repack = args.repack == "on"
+
     def launch(step: int):
         dl.track_device(r_dev.ptr, n, T, turn=step * T, energy=ring.energy,
                         rest_energy=ring.particle.rest_energy, losses=True,
                         loss_ptrs=loss_sets[step].ptrs(), stream=stream.handle,
-                        strict=args.strict)
+                        strict=args.strict, repack=repack)
 
     print(f"track: {W} warm-up and {K} timed launches of {T} turns, {n} particles",
           file=sys.stderr, flush=True)
@@ -707,6 +711,7 @@ def main() -> int:
             "turns": K * T,
             "survivors": survivors,
             "flop_per_particle_turn": float(lattice_flops(ring).sum()),
+            "repacked_turn_blocks": repack,
         },
         "roofline": {
             "bound": "fp64_valu",

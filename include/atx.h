@@ -44,12 +44,12 @@ extern "C" {
 #define ATX_CONTINUE_LOSSES 0x10u /* atx_track_batch: the loss arrays hold the map of earlier
                                      calls on the same particles; they are not reset, and
                                      particles already flagged are not recorded again */
-#define ATX_REPACK 0x20u         /* atx_track_batch: track in turn blocks of 1, 1, 2, 4, ... turns
-                                  * and, between blocks, pack each instance's surviving
-                                  * particles into its leading slots and the workgroups that
-                                  * hold them, over all instances, at the front of the grid;
-                                  * the blocks run as one hipGraph. Results are identical to
-                                  * a plain launch. */
+#define ATX_REPACK 0x20u         /* atx_track_batch, and atx_track without refpts: track in
+                                  * turn blocks of 1, 1, 2, 4, ... turns and, between blocks,
+                                  * pack each instance's surviving particles into its leading
+                                  * slots and the workgroups that hold them, over all
+                                  * instances, at the front of the grid; the blocks run as one
+                                  * hipGraph. Results are identical to a plain launch. */
 
 typedef struct atx_lattice atx_lattice;
 
