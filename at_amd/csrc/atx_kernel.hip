@@ -1677,6 +1677,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
           if (has[q] && sl >= 0) store6(rout + (base + sl) * np6 + 6 * (int64_t)p[q], 1, r[q]);
       }
       cDevElem& h = elems[e];
+      bool unchanged = false;  // no coordinate changed: the loss check would repeat the last
       if (h.kind == K_DRIFT && h.flags == 0) {
         // A drift without misalignment or apertures (DriftPass.c:16-48), half
         // the elements of a ring: tested first, from the header's first two
@@ -1686,6 +1687,12 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 #pragma unroll
         for (int q = 0; q < P; q++)
           if (!__builtin_isnan(r[q].x)) drift6(r[q], len);
+      } else if (h.kind == K_IDENTITY && h.flags == 0) {
+        // A marker or monitor without misalignment or apertures
+        // (IdentityPass.c:20-43) changes no coordinate, so its loss check
+        // would repeat the one after the element before it: only the first
+        // element of the call checks.
+        unchanged = t > 0 || e > e0;
       } else if (P == 1) {
         if (!__builtin_isnan(r[0].x)) element_lane<EXT, 1, U>(r, h, pool, nturn);
       } else {
@@ -1733,7 +1740,7 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
           }
         }
       }
-      if (check) {
+      if (check && !unchanged) {
 #pragma unroll
         for (int q = 0; q < P; q++) {
           if (losses) {
