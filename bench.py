@@ -374,6 +374,55 @@ def bench_serve(args) -> dict:
     }
 
 
+def bench_shapes(args) -> dict:
+    """The large-ring launch in every launch shape (workgroup size, particles
+    per lane, slices per iteration): ``--particles`` particles, a 2-turn
+    warm-up launch, then one timed launch of 20 turns; results do not depend
+    on the shape, only the time does."""
+    from at_amd import device, engine
+    from at_amd.batch import LossMap
+    from at_amd.flops import tracked_flops_and_passes
+    from at_amd.workloads import gaussian_beam, large_ring
+
+    ring = large_ring()
+    n, T = args.particles, 20
+    beam = gaussian_beam(n, seed=0, scale=10.0)
+    dl = engine.DeviceLattice(ring)
+    stream = device.Stream()
+    kw = dict(energy=ring.energy, rest_energy=ring.particle.rest_energy, losses=True,
+              stream=stream.handle)
+    shapes = [(b, 1, u) for b in (64, 128, 256, 512) for u in (1, 2, 4)]
+    shapes += [(b, 2, u) for b in (64, 128, 256, 512) for u in (1, 2)]
+    out = []
+    for sh in shapes:
+        dl.set_launch_shape(*sh)
+        r = device.DeviceArray.from_host(beam.T)
+        lm0, lm = LossMap.empty(n), LossMap.empty(n)
+        dl.track_device(r.ptr, n, 2, turn=0, loss_ptrs=lm0.ptrs(), **kw)
+        (e0, e1), = _pairs(1)
+        stream.synchronize()
+        e0.record(stream)
+        dl.track_device(r.ptr, n, T, turn=2, loss_ptrs=lm.ptrs(), **kw)
+        e1.record(stream)
+        stream.synchronize()
+        ms = e0.elapsed_ms(e1)
+        alive = ~lm0.flag.to_host().astype(bool)
+        fl, _ = tracked_flops_and_passes(ring, T, 2, lm.flag.to_host().astype(bool)[alive],
+                                         lm.turn.to_host()[alive], lm.elem.to_host()[alive])
+        out.append({"block": sh[0], "ppt": sh[1], "unroll": sh[2], "ms_per_turn": ms / T,
+                    "fp64_frac": fl / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS})
+        print(f"shapes: {sh}: {ms / T:.3f} ms per turn", file=sys.stderr, flush=True)
+        r.close()
+    dl.close()
+    best = min(out, key=lambda o: o["ms_per_turn"])
+    return {"metric": "large-ring launch per launch shape: ms per turn", "unit": "ms",
+            "value": best["ms_per_turn"], "higher_is_better": False, "n_gpus": 1,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"large_ring ({len(ring)} elements), {n} particles, turns "
+                                   f"2-22, fast FP64 build"},
+            "shapes": out, "best": best}
+
+
 def bench_monitors(args) -> dict:
     """Monitor-heavy tracking: the large ring with a refpt before every
     element and at the end (2,242 copies of every particle per turn, the
@@ -551,13 +600,15 @@ def bench_adjoint(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint", "monitors"),
+    ap.add_argument("--mode", choices=("track", "batch", "serve", "adjoint", "monitors",
+                                       "shapes"),
                     default="track",
                     help="track (default): the large-ring tracking line with the serving sweep "
                          "and the CPU baseline; batch: many error realisations in one launch; "
                          "serve: continuous batching of a full-size request trace; adjoint: "
                          "strength gradients; monitors: refpts at every element (HBM "
-                         "bandwidth of the copies)")
+                         "bandwidth of the copies); shapes: the large-ring launch in every "
+                         "launch shape")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-turns", type=int, default=1024)
     ap.add_argument("--shift", type=float, default=None,
@@ -627,7 +678,7 @@ def main() -> int:
     engine.set_device(0)
     if args.mode != "track":
         fn = {"batch": bench_batch, "serve": bench_serve, "adjoint": bench_adjoint,
-              "monitors": bench_monitors}[args.mode]
+              "monitors": bench_monitors, "shapes": bench_shapes}[args.mode]
         print(json.dumps(fn(args)), flush=True)
         return 0
     ring = large_ring()
