@@ -24,8 +24,8 @@ from .acceptance import (  # noqa: E402,F401
 )
 from .physics import (  # noqa: E402,F401
     DConstant, ELossMethod, element_matrices, find_elem_m66, find_m44, find_m66, find_orbit,
-    find_orbit4, find_orbit4_batch, find_orbit6, find_sync_orbit, get_energy_loss,
-    get_timelag_fromU0,
+    find_orbit4, find_orbit4_batch, find_orbit6, find_orbit6_batch, find_sync_orbit,
+    get_energy_loss, get_timelag_fromU0,
 )
 
 # Lattice methods, as pyat attaches them (physics/__init__.py, acceptance)

@@ -515,6 +515,49 @@ def element_matrices(ring, orbits=None, **kwargs):
     return _elements(ring, orbits, xy, **kwargs)
 
 
+def _close_batch(rings, xs, free, rows, lin, consts, steps, conv, maxit, strict):
+    """_close for many rings at once: every Newton iteration is one batched
+    launch of the probes of the rings not yet converged, each ring with its
+    own state and stopping rule. Returns the tracker (the caller closes it)
+    and the indices of the rings that did not converge."""
+    probe_steps = np.zeros(6)
+    probe_steps[free] = steps[free]
+    tracker = _rings(rings, strict=strict)
+    try:
+        active = list(range(len(rings)))
+        for _ in range(maxit):
+            if not active:
+                break
+            batches = [_probe_batch(xs[i], probe_steps) for i in active]
+            tracker.turn(active, [b for b, _ in batches])
+            still = []
+            for i, (probe, varied) in zip(active, batches):
+                jac = _differences(probe, probe_steps, varied)
+                residual = probe[rows, 0] - lin @ xs[i] - consts[i]
+                update = np.linalg.solve(jac[rows] - lin[:, free], residual)
+                xs[i, free] -= update
+                if not np.linalg.norm(update) <= conv:
+                    still.append(i)
+            active = still
+    except BaseException:
+        tracker.close()
+        raise
+    return tracker, active
+
+
+def _finish_batch(tracker, rings, xs, active, refpts):
+    try:
+        if active:
+            warnings.warn(AtWarning(f"Maximum number of iterations reached for rings {active}. "
+                                    "Possible non-convergence"), stacklevel=3)
+        along = None
+        if refpts is not None:
+            along = [tracker.along(i, xs[i], _refs(rings[i], refpts)) for i in range(len(rings))]
+    finally:
+        tracker.close()
+    return xs, along
+
+
 def find_orbit4_batch(rings, dp=0.0, refpts=None, *, guesses=None, strict=False, **kwargs):
     """find_orbit4 at momentum deviation ``dp`` for many rings at once (the
     error realisations of a lattice, say): every Newton iteration is one
@@ -529,32 +572,37 @@ def find_orbit4_batch(rings, dp=0.0, refpts=None, *, guesses=None, strict=False,
     xs = np.zeros((nr, 6)) if guesses is None else np.array(guesses, dtype=float).reshape(nr, 6)
     xs[:, 4] = 0.0 if dp is None else dp
     t = np.arange(4)
-    steps = np.zeros(6)
-    steps[t] = xy
-    tracker = _rings(rings, strict=strict)
-    try:
-        active = list(range(nr))
-        for _ in range(maxit):
-            if not active:
-                break
-            batches = [_probe_batch(xs[i], steps) for i in active]
-            probes = [b for b, _ in batches]
-            tracker.turn(active, probes)
-            still = []
-            for i, (probe, varied) in zip(active, batches):
-                image = probe[:, 0]
-                jac = _differences(probe, steps, varied)
-                update = np.linalg.solve(jac[t] - np.eye(4), image[t] - xs[i, t])
-                xs[i, t] -= update
-                if not np.linalg.norm(update) <= conv:
-                    still.append(i)
-            active = still
-        if active:
-            warnings.warn(AtWarning(f"Maximum number of iterations reached for rings {active}. "
-                                    "Possible non-convergence"), stacklevel=2)
-        along = None
-        if refpts is not None:
-            along = [tracker.along(i, xs[i], _refs(rings[i], refpts)) for i in range(nr)]
-    finally:
-        tracker.close()
-    return xs, along
+    tracker, active = _close_batch(rings, xs, t, t, np.eye(6)[t], np.zeros((nr, 4)),
+                                   xy * np.ones(6), conv, maxit, strict)
+    return _finish_batch(tracker, rings, xs, active, refpts)
+
+
+def find_orbit6_batch(rings, refpts=None, *, guesses=None, strict=False, **kwargs):
+    """find_orbit6 for many rings with cavities at once (orbit.py:371-458),
+    one batched launch per Newton iteration as find_orbit4_batch; each
+    ring's starting ct is its own synchronous time lag unless ``guesses``
+    are given. Returns (orbits (R, 6), orbits at refpts per ring or None)."""
+    conv, maxit, xy, dps, method = _options(kwargs, "orbit6")
+    nr = len(rings)
+    xs = np.zeros((nr, 6))
+    consts = np.zeros((nr, 6))
+    for i, ring in enumerate(rings):
+        if not _cavities(ring):
+            raise AtError(f"No cavity found in the lattice of ring {i}")
+        length = cell_length(ring)
+        f_rf = get_rf_frequency(ring)
+        harmonic = round(f_rf * length / beta(ring) / CLIGHT)
+        consts[i, 5] = beta(ring) * CLIGHT * harmonic / f_rf - length
+        if guesses is None:
+            try:
+                _, lag = get_timelag_fromU0(ring, method=method)
+            except AtError as exc:
+                raise AtError("Could not determine the initial synchronous phase") from exc
+            xs[i, 5] = -lag
+        else:
+            xs[i] = np.asarray(guesses[i], dtype=float)
+    steps = xy * _TRANSVERSE + dps * (1.0 - _TRANSVERSE)
+    every = np.arange(6)
+    tracker, active = _close_batch(rings, xs, every, every, np.eye(6), consts, steps, conv,
+                                   maxit, strict)
+    return _finish_batch(tracker, rings, xs, active, refpts)

@@ -12,6 +12,7 @@ from test_physics_acceptance import (  # noqa: F401  (collected here with the en
     test_find_m44_symplectic,
     test_find_m66_known_answer,
     test_find_orbit4_batch_equals_one_by_one,
+    test_find_orbit6_batch_equals_one_by_one,
     test_find_orbit4_zero_and_propagation,
     test_find_orbit6_known_answer,
     test_grid_modes_bracket_the_recursive_boundary,

@@ -168,3 +168,15 @@ def test_find_orbit4_batch_equals_one_by_one(hooks, hmba):
     orbits2, none = physics.find_orbit4_batch(rings, 1e-4, guesses=orbits)
     assert none is None
     np.testing.assert_allclose(orbits2, orbits, rtol=0, atol=1e-12)
+
+
+def test_find_orbit6_batch_equals_one_by_one(hooks, hmba):
+    """The 6-D closed orbits of several radiating rings in batched launches
+    equal find_orbit6's ring by ring, refpt orbits included."""
+    rings = _realisations(hmba.radiation_on(copy=True), 3)
+    orbits, along = physics.find_orbit6_batch(rings, [0, 40])
+    for i, r in enumerate(rings):
+        o, pts = physics.find_orbit6(r, [0, 40])
+        np.testing.assert_array_equal(orbits[i], o)
+        np.testing.assert_array_equal(along[i], pts)
+    assert np.abs(orbits[:, 4]).max() > 0  # radiation: an energy offset
