@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .lattice import (Aperture, Corrector, Dipole, Drift, Lattice, Multipole, Octupole,
+from .lattice import (Aperture, Corrector, Dipole, Drift, Lattice, Octupole,
                       Particle, Quadrupole, RFCavity, Sextupole, ThinMultipole)
 
 SIGMA0 = np.array([1e-4, 1e-5, 1e-5, 1e-6, 1e-3, 1e-3])

@@ -501,7 +501,6 @@ def bench_adjoint(args) -> dict:
     import numpy as np
     import torch
 
-    from at_amd import lattice as L
     from at_amd.adjoint import Problem, TrackingLoss, sweep_plan
     from at_amd.workloads import gaussian_beam, large_ring
 

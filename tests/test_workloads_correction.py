@@ -6,7 +6,7 @@ oracle doing the tracking (tests/oracle_hooks.py)."""
 import numpy as np
 import pytest
 
-from at_amd import physics, workloads
+from at_amd import workloads
 from at_amd.lattice import Corrector, Sextupole
 from oracle import pyoracle as po
 
