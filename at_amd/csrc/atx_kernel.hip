@@ -625,10 +625,17 @@ __device__ __forceinline__ void rad_bnd_kick_ang(R6& r, Ang& g, cdouble* a, cdou
 // Entry and exit of a thick multipole, per particle: FieldScaling, T1/R1,
 // apertures, bend edge, quadrupole fringe (StrMPoleSymplectic4Pass.c:57-78,
 // 96-104; BndMPoleSymplectic4Pass.c:75-96, 114-122).
+// One test for the transforms and apertures an unmisaligned magnet without
+// apertures has none of, before the tests of each.
+constexpr uint32_t F_ENTRY_ANY = F_T1 | F_R1 | F_RAP | F_EAP;
+constexpr uint32_t F_EXIT_ANY = F_R2 | F_T2 | F_RAP | F_EAP | F_SCALE;
+
 template <bool BEND>
 __device__ __forceinline__ void mpole_entry(R6& r, cDevElem& h, cdouble* pool, cdouble* s) {
-  entry_xform(r, h, pool);
-  apertures(r, h, pool);
+  if (h.flags & F_ENTRY_ANY) {
+    entry_xform(r, h, pool);
+    apertures(r, h, pool);
+  }
   if (BEND) edge(r, h.edge & 0xff, s[S_IRHO], s[S_FX_IN], s[S_FY_IN], s[S_EDGE_IN], s[S_CORR_IN], 1.0);
   if (h.fringe & 0xff) quad_fringe_in(r, h, pool, s[S_B2]);
 }
@@ -637,9 +644,11 @@ template <bool BEND>
 __device__ __forceinline__ void mpole_exit(R6& r, cDevElem& h, cdouble* pool, cdouble* s) {
   if (h.fringe & 0xff00) quad_fringe_out(r, h, pool, s[S_B2]);
   if (BEND) edge(r, (h.edge >> 8) & 0xff, s[S_IRHO], s[S_FX_OUT], s[S_FY_OUT], s[S_EDGE_OUT], s[S_CORR_OUT], -1.0);
-  apertures(r, h, pool);
-  exit_xform(r, h, pool);
-  if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+  if (h.flags & F_EXIT_ANY) {
+    apertures(r, h, pool);
+    exit_xform(r, h, pool);
+    if (h.flags & F_SCALE) change_pref(r, s[S_INVSCALE]);
+  }
 }
 
 // StrMPoleSymplectic4Pass.c:57-104 and BndMPoleSymplectic4Pass.c:75-122 for
