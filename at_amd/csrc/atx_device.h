@@ -84,7 +84,7 @@ enum Slot : int32_t {
   // RF cavity
   S_RF_HALF = 0, S_RF_NV = 1, S_RF_TWOPIF = 2, S_RF_LAG = 3, S_RF_PHILAG = 4, S_RF_HFT0 = 5,
   // corrector
-  S_C_XK = 0, S_C_YK = 1, S_C_LEN = 2,
+  S_C_XK = 0, S_C_YK = 1, S_C_LEN = 2, S_C_Q0 = 3,  // S_C_Q0 = xk*xk/3 + yk*yk/3
   // drift
   S_D_LEN = 0
 };

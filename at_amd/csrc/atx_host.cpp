@@ -389,6 +389,9 @@ int compile_elem(const HostElem& e, int idx, double energy, double rest, double 
       sc[S_C_XK] = (*ka)[0];
       sc[S_C_YK] = (*ka)[1];
       sc[S_C_LEN] = len;
+      // the constant leading terms of the thick corrector's path-length sum
+      // (CorrectorPass.c:55-89), added first there, in the same order
+      sc[S_C_Q0] = sc[S_C_XK] * sc[S_C_XK] / 3.0 + sc[S_C_YK] * sc[S_C_YK] / 3.0;
       sc[S_SCALE] = s;
       sc[S_INVSCALE] = 1.0 / s;
       if (s != 1.0) h.flags |= F_SCALE;

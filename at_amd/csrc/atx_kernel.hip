@@ -132,10 +132,28 @@ __device__ __forceinline__ void exit_xform(R6& r, cDevElem& h, cdouble* pool) {
   if (h.flags & F_T2) add_t(r, pool + h.t2);
 }
 
+#if !ATX_STRICT
+// Fast build: 1/d from the hardware reciprocal estimate (v_rcp_f64) and two
+// Newton steps q <- q + q (1 - d q), each of which squares the relative
+// error of the estimate; the result is within an ulp of 1/d.
+__device__ __forceinline__ double fast_rcp(double d) {
+  double q = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, q, 1.0);
+  q = __builtin_fma(q, e, q);
+  e = __builtin_fma(-d, q, 1.0);
+  return __builtin_fma(q, e, q);
+}
+#endif
+
 // Full drift with its own 1/(1+delta) (atlalib.c:59-68, driftkickrad.c:36-45,
-// diff_drift.c:30-43 share this arithmetic).
+// diff_drift.c:30-43 share this arithmetic); the fast build takes fast_rcp
+// for the division.
 __device__ __forceinline__ void drift6(R6& r, double len) {
+#if ATX_STRICT
   const double pn = 1.0 / (1.0 + r.dp);
+#else
+  const double pn = fast_rcp(1.0 + r.dp);
+#endif
   const double nl = len * pn;
   r.x += nl * r.px;
   r.y += nl * r.py;
@@ -240,17 +258,6 @@ __device__ __forceinline__ void rad_drift(R6& r, double len) {
 }
 
 #if !ATX_STRICT
-// Fast build: 1/d from the hardware reciprocal estimate (v_rcp_f64) and two
-// Newton steps q <- q + q (1 - d q), each of which squares the relative
-// error of the estimate; the result is within an ulp of 1/d.
-__device__ __forceinline__ double fast_rcp(double d) {
-  double q = __builtin_amdgcn_rcp(d);
-  double e = __builtin_fma(-d, q, 1.0);
-  q = __builtin_fma(q, e, q);
-  e = __builtin_fma(-d, q, 1.0);
-  return __builtin_fma(q, e, q);
-}
-
 // One Newton step: for a reciprocal that only scales the radiated energy (a
 // relative change of 1e-8 or less per kick), where an error of the squared
 // estimate error is far below the rounding of delta.
@@ -663,7 +670,11 @@ __device__ __forceinline__ void mpole(R6* r, cDevElem& h, cdouble* pool) {
 #pragma unroll
   for (int q = 0; q < P; q++) {
     if (h.flags & F_SCALE) change_pref(r[q], s[S_SCALE]);
+#if ATX_STRICT
     pn[q] = 1.0 / (1.0 + r[q].dp);  // before T1/R1, as the reference hoists it
+#else
+    pn[q] = fast_rcp(1.0 + r[q].dp);  // before T1/R1, as the reference hoists it
+#endif
     nl1[q] = s[S_L1] * pn[q];
     nl2[q] = s[S_L2] * pn[q];
     mpole_entry<BEND>(r[q], h, pool, s);
@@ -714,7 +725,7 @@ __device__ __forceinline__ void mpole(R6* r, cDevElem& h, cdouble* pool) {
     double nh1[P], nh2[P];  // delta is constant through the slices
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      const double h2 = 0.5 / (1.0 + r[q].dp);
+      const double h2 = 0.5 * fast_rcp(1.0 + r[q].dp);
       nh1[q] = nl1[q] * h2;
       nh2[q] = nl2[q] * h2;
     }
@@ -1068,11 +1079,16 @@ __device__ __forceinline__ void corrector(R6& r, cDevElem& h, cdouble* pool) {
     r.px += xk;
     r.py += yk;
   } else {
+#if ATX_STRICT
     const double pn = 1.0 / (1.0 + r.dp);
+#else
+    const double pn = fast_rcp(1.0 + r.dp);
+#endif
     const double nl = s[S_C_LEN] * pn;
     entry_xform(r, h, pool);
     apertures(r, h, pool);
-    r.ct += nl * pn * (xk * xk / 3.0 + yk * yk / 3.0 + r.px * r.px + r.py * r.py + r.px * xk + r.py * yk) / 2.0;
+    // s[S_C_Q0] = xk*xk/3 + yk*yk/3, the sum's first two terms
+    r.ct += nl * pn * (s[S_C_Q0] + r.px * r.px + r.py * r.py + r.px * xk + r.py * yk) / 2.0;
     r.x += nl * (r.px + xk / 2.0);
     r.px += xk;
     r.y += nl * (r.py + yk / 2.0);
