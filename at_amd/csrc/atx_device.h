@@ -16,6 +16,10 @@
 
 namespace atx {
 
+// XCDs of the device: the hardware deals the workgroups of a launch to them
+// in turn (workgroup i to XCD i mod kXcds), and each has its own L2.
+constexpr int kXcds = 8;
+
 // Dispatch kinds of the kernel switch (one per in-scope pass method).
 enum Kind : int32_t {
   K_IDENTITY = 0,

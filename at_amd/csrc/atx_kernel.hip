@@ -1528,17 +1528,32 @@ __device__ __forceinline__ R6 lost_state() { return R6{__builtin_nan(""), 0.0, 0
 // other workgroups end at once.
 template <bool EXT, int P, int U>
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst,
-                                                    const int32_t* __restrict__ live_begin) {
+                                                    int nblocks, const int32_t* __restrict__ live_begin) {
   static_assert(P == 1 || P == 2, "one or two particles per lane");
   extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
-  // Instance of this workgroup: the last whose first block is <= blockIdx.x.
+  // Workgroup order of a launch over several instances: the hardware deals
+  // workgroup i to XCD i mod kXcds, and each XCD has its own L2. The
+  // workgroups of the launch (nblocks, or the live ones of a repacked turn
+  // block) are dealt in kXcds contiguous chunks, one per XCD, so that the
+  // instances (lattices) of one chunk are read through one L2 rather than
+  // every L2 holding every lattice. The grid is kXcds * ceil(nblocks /
+  // kXcds); the map is a bijection onto the workgroups below that count,
+  // and the rest end at once. One instance keeps the hardware order.
+  const int nlog = live_begin ? live_begin[ninst] : nblocks;
+  const int hw = (int)blockIdx.x;
+  int bid = hw;
+  if (ninst > 1) {
+    const int chunk = (nlog + kXcds - 1) / kXcds;
+    if (hw >= kXcds * chunk) return;
+    bid = (hw % kXcds) * chunk + hw / kXcds;
+  }
+  if (bid >= nlog) return;
+  // Instance of this workgroup: the last whose first block is <= bid.
   int lo = 0, hi = ninst - 1;
-  const int bid = (int)blockIdx.x;
   int wbegin;
   int G = (int)blockDim.x * P;  // particle slots per workgroup
   if (live_begin) {
-    if (bid >= live_begin[ninst]) return;
     G = live_begin[ninst + 1];
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -1803,12 +1818,15 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
                                  int ext, const int32_t* live_begin, hipStream_t stream) {
   const size_t lds = atx::ATX_NS::compact_lds_bytes(block * ATX_PPT);
+  // whole rounds of XCDs: the kernel's workgroup order needs the grid to
+  // cover kXcds * ceil(n / kXcds) for every n <= nblocks
+  const int grid = (nblocks + atx::kXcds - 1) / atx::kXcds * atx::kXcds;
   if (ext)
-    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<true, ATX_PPT, ATX_UNROLL>), dim3(nblocks),
-                       dim3(block), lds, stream, d_insts, ninst, live_begin);
+    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<true, ATX_PPT, ATX_UNROLL>), dim3(grid),
+                       dim3(block), lds, stream, d_insts, ninst, nblocks, live_begin);
   else
-    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<false, ATX_PPT, ATX_UNROLL>), dim3(nblocks),
-                       dim3(block), lds, stream, d_insts, ninst, live_begin);
+    hipLaunchKernelGGL((atx::ATX_NS::track_kernel<false, ATX_PPT, ATX_UNROLL>), dim3(grid),
+                       dim3(block), lds, stream, d_insts, ninst, nblocks, live_begin);
   return hipGetLastError();
 }
 
