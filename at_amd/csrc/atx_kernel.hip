@@ -586,13 +586,27 @@ __device__ __forceinline__ void rad_str_kick_ang(R6& r, Ang& g, cdouble* a, cdou
   const double cr = __builtin_fma(im, g.yp, -re * g.xp);
   const double b2 = __builtin_fma(cr, cr, __builtin_fma(re, re, im * im));
   // 1 - (s2/2)(1 - s2 + s2^2) by Horner in s2
-  double f = __builtin_fma(s2, __builtin_fma(s2, __builtin_fma(-0.5, s2, 0.5), -0.5), 1.0);
+  const double f = __builtin_fma(s2, __builtin_fma(s2, __builtin_fma(-0.5, s2, 0.5), -0.5), 1.0);
+  const double oo = opd * opd;
+  const double d0 = r.dp;
+  double dp = __builtin_fma(-radk, (b2 * f) * oo, d0);
+  double opd2 = 1.0 + dp;
+  double e = __builtin_fma(-opd2, g.pn, 1.0);
+  double pn = __builtin_fma(g.pn, __builtin_fma(e, e, e), g.pn);
+  // one vote for both exceptions of the wave: a wide angle (the factor by
+  // division) and a large radiated fraction (the reciprocal from fast_rcp)
   const bool wide = s2 > 1e-4;
-  if (__builtin_expect(__ballot(wide) != 0, 0)) {
-    if (wide) f = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);
+  if (__builtin_expect(__ballot(wide || __builtin_fabs(e) > 2e-6) != 0, 0)) {
+    if (wide) {
+      const double fw = __builtin_fma(0.5, s2, 1.0) * rcp1(1.0 + s2);
+      dp = __builtin_fma(-radk, (b2 * fw) * oo, d0);
+      opd2 = 1.0 + dp;
+      e = __builtin_fma(-opd2, g.pn, 1.0);
+      pn = __builtin_fma(g.pn, __builtin_fma(e, e, e), g.pn);
+    }
+    if (__builtin_fabs(e) > 2e-6) pn = fast_rcp(opd2);
   }
-  r.dp = __builtin_fma(-radk, (b2 * f) * (opd * opd), r.dp);
-  const double pn = rcp_after_kick(1.0 + r.dp, g.pn);
+  r.dp = dp;
   const double klpn = kl * pn;
   g.xp = __builtin_fma(-re, klpn, g.xp);
   g.yp = __builtin_fma(im, klpn, g.yp);
