@@ -22,7 +22,8 @@
 // Tracking kernels, one per build and launch shape (particles per lane x
 // slices per loop iteration), compiled from atx_kernel.hip as separate objects.
 #define ATX_TRACK_DECL(name)                                                                  \
-  extern "C" hipError_t name(const atx::Instance*, int, int, int, int, const int32_t*, hipStream_t);
+  extern "C" hipError_t name(const atx::Instance*, int, int, int, int, int, const int32_t*,       \
+                             hipStream_t);
 ATX_TRACK_DECL(atx_launch_track_strict_p1_u1)
 ATX_TRACK_DECL(atx_launch_track_strict_p1_u2)
 ATX_TRACK_DECL(atx_launch_track_strict_p1_u4)
@@ -693,7 +694,20 @@ constexpr int kPpt = 1;
 constexpr int kUnrollFast = 2;
 constexpr int kUnrollStrict = 1;
 
-typedef hipError_t (*TrackLaunch)(const Instance*, int, int, int, int, const int32_t*, hipStream_t);
+typedef hipError_t (*TrackLaunch)(const Instance*, int, int, int, int, int, const int32_t*,
+                                  hipStream_t);
+
+// XCD-aware workgroup order (the kernel deals the launch's workgroups to the
+// XCDs in contiguous chunks, one L2 per group of instances): for launches
+// over several instances that all run the same number of turns. Items of
+// unequal turn counts keep the hardware order, which spreads every item
+// over all XCDs.
+bool xcd_order(const Instance* insts, size_t n) {
+  if (n < 2) return false;
+  for (size_t i = 1; i < n; i++)
+    if (insts[i].nturns != insts[0].nturns) return false;
+  return true;
+}
 
 // The tracking kernel of a build and a (valid) launch shape.
 TrackLaunch track_launcher(bool strict, const atx_launch_shape& sh) {
@@ -813,7 +827,8 @@ int launch_instances(std::vector<Instance>& insts, const std::vector<const std::
   hipError_t le = hipSuccess;
   if (ce == hipSuccess)
     le = track_launcher(strict, shape)((const Instance*)d_scratch, (int)insts.size(), nblocks, block,
-                                       ext ? 1 : 0, nullptr, stream);
+                                       ext ? 1 : 0, xcd_order(insts.data(), insts.size()) ? 1 : 0,
+                                       nullptr, stream);
   hipError_t fe = hipFreeAsync(d_scratch, stream);
   if (ce != hipSuccess) return fail(ATX_E_RUNTIME, std::string("upload: ") + hipGetErrorString(ce));
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
@@ -865,6 +880,7 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
   const int fine = 64 * ppt;       // ... when the survivors run one wave per workgroup
   reap_graphs();
   const size_t ninst = insts.size();
+  const bool xcd = xcd_order(insts.data(), ninst);
   int nblocks = 0;
   int nblocks_fine = 0;  // grid of the later blocks: room for 64 particles per workgroup
   int maxt = 0;
@@ -951,7 +967,8 @@ int launch_repacked(std::vector<Instance>& insts, const std::vector<const std::v
       if (le != hipSuccess) break;
       const int32_t* lb = k > 0 ? live : nullptr;
       const int grid = k > 0 ? nblocks_fine : nblocks;
-      le = track_launcher(strict, shape)(t, (int)ninst, grid, block, ext ? 1 : 0, lb, cap);
+      le = track_launcher(strict, shape)(t, (int)ninst, grid, block, ext ? 1 : 0, xcd ? 1 : 0, lb,
+                                         cap);
       if (le == hipSuccess && k + 1 < nblk)
         le = atx_launch_repack((const Instance*)(d + (size_t)(k + 1) * tab), (int)ninst, k == 0 ? 1 : 0, cap);
     }
@@ -1534,7 +1551,8 @@ int atx_track_batch_noalloc(const atx_batch_item* items, int32_t nitems, int32_t
     HIPCHK(hipGetLastError());
   }
   const hipError_t le = track_launcher(strict, shape)(d_tab, n, nblocks, shape.block, ext ? 1 : 0,
-                                                      nullptr, stream);
+                                                      xcd_order(tab, (size_t)n) ? 1 : 0, nullptr,
+                                                      stream);
   if (le != hipSuccess) return fail(ATX_E_RUNTIME, std::string("kernel launch: ") + hipGetErrorString(le));
   return ATX_OK;
 }

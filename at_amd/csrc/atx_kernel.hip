@@ -1542,22 +1542,26 @@ __device__ __forceinline__ R6 lost_state() { return R6{__builtin_nan(""), 0.0, 0
 // other workgroups end at once.
 template <bool EXT, int P, int U>
 __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__ insts_g, int ninst,
-                                                    int nblocks, const int32_t* __restrict__ live_begin) {
+                                                    int nblocks, int xcd_order,
+                                                    const int32_t* __restrict__ live_begin) {
   static_assert(P == 1 || P == 2, "one or two particles per lane");
   extern __shared__ double lds[];
   cInstance* insts = (cInstance*)insts_g;
-  // Workgroup order of a launch over several instances: the hardware deals
-  // workgroup i to XCD i mod kXcds, and each XCD has its own L2. The
-  // workgroups of the launch (nblocks, or the live ones of a repacked turn
-  // block) are dealt in kXcds contiguous chunks, one per XCD, so that the
-  // instances (lattices) of one chunk are read through one L2 rather than
-  // every L2 holding every lattice. The grid is kXcds * ceil(nblocks /
-  // kXcds); the map is a bijection onto the workgroups below that count,
-  // and the rest end at once. One instance keeps the hardware order.
+  // Workgroup order (xcd_order: a launch over several instances of equal
+  // turn counts): the hardware deals workgroup i to XCD i mod kXcds, and
+  // each XCD has its own L2. The workgroups of the launch (nblocks, or the
+  // live ones of a repacked turn block) are dealt in kXcds contiguous
+  // chunks, one per XCD, so that the instances (lattices) of one chunk are
+  // read through one L2 rather than every L2 holding every lattice. The
+  // grid is kXcds * ceil(nblocks / kXcds); the map is a bijection onto the
+  // workgroups below that count, and the rest end at once. Otherwise the
+  // hardware order stands: one lattice is read alike by every XCD, and
+  // items of unequal turn counts, dealt in chunks, would load the XCDs
+  // unevenly.
   const int nlog = live_begin ? live_begin[ninst] : nblocks;
   const int hw = (int)blockIdx.x;
   int bid = hw;
-  if (ninst > 1) {
+  if (xcd_order) {
     const int chunk = (nlog + kXcds - 1) / kXcds;
     if (hw >= kXcds * chunk) return;
     bid = (hw % kXcds) * chunk + hw / kXcds;
@@ -1830,17 +1834,18 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
 // The kernel of this object's launch shape: nblocks workgroups of `block`
 // lanes, block * ATX_PPT particle slots each.
 extern "C" hipError_t ATX_LAUNCH(const atx::Instance* d_insts, int ninst, int nblocks, int block,
-                                 int ext, const int32_t* live_begin, hipStream_t stream) {
+                                 int ext, int xcd_order, const int32_t* live_begin,
+                                 hipStream_t stream) {
   const size_t lds = atx::ATX_NS::compact_lds_bytes(block * ATX_PPT);
   // whole rounds of XCDs: the kernel's workgroup order needs the grid to
   // cover kXcds * ceil(n / kXcds) for every n <= nblocks
   const int grid = (nblocks + atx::kXcds - 1) / atx::kXcds * atx::kXcds;
   if (ext)
     hipLaunchKernelGGL((atx::ATX_NS::track_kernel<true, ATX_PPT, ATX_UNROLL>), dim3(grid),
-                       dim3(block), lds, stream, d_insts, ninst, nblocks, live_begin);
+                       dim3(block), lds, stream, d_insts, ninst, nblocks, xcd_order, live_begin);
   else
     hipLaunchKernelGGL((atx::ATX_NS::track_kernel<false, ATX_PPT, ATX_UNROLL>), dim3(grid),
-                       dim3(block), lds, stream, d_insts, ninst, nblocks, live_begin);
+                       dim3(block), lds, stream, d_insts, ninst, nblocks, xcd_order, live_begin);
   return hipGetLastError();
 }
 
