@@ -18,7 +18,7 @@ bounded sample of the same workload on the host cores.
     python bench.py [--steps 10] [--warmup 1] [--turns-per-step 100]
     python bench.py --mode batch|serve|adjoint|monitors   (the other workloads, one line each)
 
-The default line also carries ``serving``: one seeded trace of 48 requests
+The default line also carries ``serving``: one seeded trace of 96 requests
 (error realisations resident on the device, N log-uniform on [2e3, 2e5],
 turn counts scaled to {4, 16, 64} so that the offered loads 5, 15 and 50
 requests/s bracket the capacity of both modes), served by continuous
@@ -620,7 +620,7 @@ def main() -> int:
     ap.add_argument("--turn-block", type=int, default=128)
     ap.add_argument("--no-serve", action="store_true",
                     help="track: leave out the serving sweep")
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-rates", default="5,15,50",
                     help="track: offered loads of the serving sweep [requests/s]")
     ap.add_argument("--serve-turns", default="4,16,64",
