@@ -140,3 +140,39 @@ def test_repacked_launch_continues_losses():
         np.testing.assert_array_equal(two[i].to_host(), one[i].to_host())
         for x, y in zip(_lm_arrays(lm_two[i]), _lm_arrays(lm_one[i])):
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("repack", [False, True], ids=["plain", "repacked"])
+def test_equal_turn_batch_in_xcd_order_equals_single_launches(repack):
+    """A batch whose instances all run the same turns is launched with its
+    workgroups dealt to the XCDs in contiguous chunks (a workgroup-index
+    map in the kernel, over the launch's workgroups or the live ones of
+    each repacked turn block). Eleven instances of sizes from part of one
+    workgroup to several, so that the chunks split instances and the
+    workgroup counts are not multiples of the XCD count: every instance
+    equals its own single-lattice launch bit for bit (coordinates and loss
+    map)."""
+
+    base = large_ring(ncells=2, cavity=True)
+    sizes = [40, 900, 64, 333, 1500, 65, 700, 128, 2100, 5, 450]
+    lats = [error_realisation(base, 500 + i) for i in range(len(sizes))]
+    dls = [engine.DeviceLattice(l) for l in lats]
+    energy, rest = lats[0].energy, lats[0].particle.rest_energy
+    parts = [da_polar_grid(n, xmax=12e-3, ymax=5e-3) for n in sizes]
+    nturns = 24
+    print(f"xcd-order batch: {len(sizes)} instances, repack={repack}", flush=True)
+    tens = [DeviceArray.from_host(p.T) for p in parts]
+    lms = track_batch(dls, tens, nturns, energy=energy, rest_energy=rest, block_size=64,
+                      repack=repack)
+    device.synchronize()
+    nlost = 0
+    for i, p in enumerate(parts):
+        single = p.copy(order="F")
+        _, lm1 = dls[i].track(single, nturns, None, energy=energy, rest_energy=rest, losses=True)
+        np.testing.assert_array_equal(tens[i].to_host().T, single, err_msg=f"item {i}")
+        flag, turn, elem, coord = _lm_arrays(lms[i])
+        np.testing.assert_array_equal(flag, lm1["islost"], err_msg=f"item {i}")
+        np.testing.assert_array_equal(turn, lm1["turn"], err_msg=f"item {i}")
+        np.testing.assert_array_equal(elem, lm1["elem"], err_msg=f"item {i}")
+        nlost += int(flag.sum())
+    assert nlost > 0
