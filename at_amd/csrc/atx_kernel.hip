@@ -1553,11 +1553,12 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
   // live ones of a repacked turn block) are dealt in kXcds contiguous
   // chunks, one per XCD, so that the instances (lattices) of one chunk are
   // read through one L2 rather than every L2 holding every lattice. The
-  // grid is kXcds * ceil(nblocks / kXcds); the map is a bijection onto the
-  // workgroups below that count, and the rest end at once. Otherwise the
-  // hardware order stands: one lattice is read alike by every XCD, and
-  // items of unequal turn counts, dealt in chunks, would load the XCDs
-  // unevenly.
+  // grid is kXcds * ceil(nblocks / kXcds) workgroups; the first kXcds *
+  // ceil(nlog / kXcds) of them map one to one onto the logical workgroups
+  // below that count, and those past nlog, like the rest, end at once. The
+  // hardware order stands otherwise: one lattice is read alike by every
+  // XCD, and items of unequal turn counts, dealt in chunks, would load the
+  // XCDs unevenly.
   const int nlog = live_begin ? live_begin[ninst] : nblocks;
   const int hw = (int)blockIdx.x;
   int bid = hw;
