@@ -11,13 +11,19 @@ that do not depend on the beam size:
   launch size change nothing in its arithmetic;
 * the strict build equals the oracle bit for bit on that sample;
 * turns compose: 3 turns in one call equal 1 turn and then 2 more from
-  turn 1, bit for bit, including the loss records."""
+  turn 1, bit for bit, including the loss records.
+
+The batched workload at its full size (64 orbit-corrected error
+realisations of the large ring, DA grids of 1,400,590 particles in all, one
+repacked launch in XCD order) is checked the same way: sampled instances
+equal their own single-lattice launches bit for bit."""
 
 import numpy as np
 import pytest
 
-from at_amd import engine
-from at_amd.workloads import gaussian_beam, large_ring
+from at_amd import device, engine
+from at_amd.batch import track_batch
+from at_amd.workloads import batched_da_workload, gaussian_beam, large_ring
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -91,3 +97,29 @@ def test_full_beam_turns_compose(setup, strict):
         merged = np.where(first, b1[k], b2[k])
         np.testing.assert_array_equal(a[k], merged, err_msg=k)
     np.testing.assert_array_equal(a["islost"].astype(bool), first | b2["islost"].astype(bool))
+
+
+def test_batched_realisations_full_size_equal_single_launches():
+    work = batched_da_workload(ninst=64, seed=7, shift=None)
+    lats = [l for l, _ in work]
+    parts = [p for _, p in work]
+    print(f"full-size batch: {len(lats)} instances, {sum(p.shape[1] for p in parts)} particles",
+          flush=True)
+    dls = [engine.DeviceLattice(l) for l in lats]
+    e, rest = lats[0].energy, lats[0].particle.rest_energy
+    tens = [device.DeviceArray.from_host(p.T) for p in parts]
+    nturns = 6
+    lms = track_batch(dls, tens, nturns, energy=e, rest_energy=rest, block_size=64, ppt=1,
+                      unroll=2, repack=True)
+    device.synchronize()
+    nlost = 0
+    for i in (0, 17, 40, 63):
+        alone = np.asfortranarray(parts[i].copy())
+        _, lm1 = dls[i].track(alone, nturns, None, energy=e, rest_energy=rest, losses=True)
+        np.testing.assert_array_equal(tens[i].to_host().T, alone, err_msg=f"instance {i}")
+        flag = lms[i].flag.to_host().astype(bool)
+        np.testing.assert_array_equal(flag, lm1["islost"], err_msg=f"instance {i}")
+        np.testing.assert_array_equal(lms[i].turn.to_host(), lm1["turn"], err_msg=f"instance {i}")
+        np.testing.assert_array_equal(lms[i].elem.to_host(), lm1["elem"], err_msg=f"instance {i}")
+        nlost += int(flag.sum())
+    assert nlost > 0
