@@ -9,7 +9,9 @@ that do not depend on the beam size:
   launch, bit for bit (coordinates and loss records), in both builds --
   the lost-particle compaction, the workgroup a particle lands in and the
   launch size change nothing in its arithmetic;
-* the strict build equals the oracle bit for bit on that sample;
+* on that sample the strict build equals the oracle bit for bit, and the
+  fast build has the oracle's loss records and its coordinates within
+  1e-11;
 * turns compose: 3 turns in one call equal 1 turn and then 2 more from
   turn 1, bit for bit, including the loss records.
 
@@ -71,12 +73,17 @@ def test_full_beam_equals_particles_alone_and_oracle(setup, strict):
     nlost = int(rf["islost"].sum())
     print(f"full size: {nlost} of {N} lost in {TURNS} turns; sample of {sample.size}", flush=True)
     assert nlost > 0 and int(rp["islost"].sum()) == nlost
+    ref = np.asfortranarray(beam[:, sample].copy())
+    _, lmo = po.atpass(ring, ref, TURNS, losses=True)
+    for k in ("islost", "turn", "elem"):
+        np.testing.assert_array_equal(rp[k], np.asarray(lmo[k]), err_msg=k)
     if strict:
-        ref = np.asfortranarray(beam[:, sample].copy())
-        _, lmo = po.atpass(ring, ref, TURNS, losses=True)
         np.testing.assert_array_equal(part, ref)
-        for k in ("islost", "turn", "elem"):
-            np.testing.assert_array_equal(rp[k], np.asarray(lmo[k]), err_msg=k)
+    else:
+        # the fast build's bar: 1e-12 per coordinate after one turn; 1e-11
+        # after these three
+        surv = ~rp["islost"]
+        np.testing.assert_allclose(part[:, surv], ref[:, surv], rtol=0, atol=1e-11)
 
 
 @pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
