@@ -18,7 +18,8 @@ that do not depend on the beam size:
 The batched workload at its full size (64 orbit-corrected error
 realisations of the large ring, DA grids of 1,400,590 particles in all, one
 repacked launch in XCD order) is checked the same way: sampled instances
-equal their own single-lattice launches bit for bit."""
+equal their own single-lattice launches bit for bit in both builds, and in
+the strict build a sample of one instance equals the oracle."""
 
 import numpy as np
 import pytest
@@ -106,8 +107,14 @@ def test_full_beam_turns_compose(setup, strict):
     np.testing.assert_array_equal(a["islost"].astype(bool), first | b2["islost"].astype(bool))
 
 
-def test_batched_realisations_full_size_equal_single_launches():
-    work = batched_da_workload(ninst=64, seed=7, shift=None)
+@pytest.fixture(scope="module")
+def realisations():
+    return batched_da_workload(ninst=64, seed=7, shift=None)
+
+
+@pytest.mark.parametrize("strict", [True, False], ids=["strict", "fast"])
+def test_batched_realisations_full_size_equal_single_launches(realisations, strict):
+    work = realisations
     lats = [l for l, _ in work]
     parts = [p for _, p in work]
     print(f"full-size batch: {len(lats)} instances, {sum(p.shape[1] for p in parts)} particles",
@@ -117,16 +124,26 @@ def test_batched_realisations_full_size_equal_single_launches():
     tens = [device.DeviceArray.from_host(p.T) for p in parts]
     nturns = 6
     lms = track_batch(dls, tens, nturns, energy=e, rest_energy=rest, block_size=64, ppt=1,
-                      unroll=2, repack=True)
+                      unroll=1 if strict else 2, repack=True, strict=strict)
     device.synchronize()
     nlost = 0
     for i in (0, 17, 40, 63):
         alone = np.asfortranarray(parts[i].copy())
-        _, lm1 = dls[i].track(alone, nturns, None, energy=e, rest_energy=rest, losses=True)
+        _, lm1 = dls[i].track(alone, nturns, None, energy=e, rest_energy=rest, losses=True,
+                              strict=strict)
         np.testing.assert_array_equal(tens[i].to_host().T, alone, err_msg=f"instance {i}")
         flag = lms[i].flag.to_host().astype(bool)
         np.testing.assert_array_equal(flag, lm1["islost"], err_msg=f"instance {i}")
         np.testing.assert_array_equal(lms[i].turn.to_host(), lm1["turn"], err_msg=f"instance {i}")
         np.testing.assert_array_equal(lms[i].elem.to_host(), lm1["elem"], err_msg=f"instance {i}")
         nlost += int(flag.sum())
+        if strict and i == 0:
+            # a sample of the instance against the oracle, bit for bit
+            pick = np.unique(np.concatenate([np.arange(0, parts[i].shape[1], 97),
+                                             np.flatnonzero(flag)]))
+            ref = np.asfortranarray(parts[i][:, pick].copy())
+            _, lmo = po.atpass(lats[i], ref, nturns, losses=True)
+            np.testing.assert_array_equal(alone[:, pick], ref)
+            for k in ("islost", "turn", "elem"):
+                np.testing.assert_array_equal(lm1[k][pick], np.asarray(lmo[k]), err_msg=k)
     assert nlost > 0
