@@ -1744,8 +1744,24 @@ __global__ void __launch_bounds__(512) track_kernel(const Instance* __restrict__
         // skips a particle whose x is NaN (at.c:142-161).
         const double len = drift_length(h);
 #pragma unroll
-        for (int q = 0; q < P; q++)
+        for (int q = 0; q < P; q++) {
+#if ATX_STRICT
           if (!__builtin_isnan(r[q].x)) drift6(r[q], len);
+#else
+          // the fast build skips a NaN-x lane by a zero step instead of
+          // selecting its old coordinates: x stays NaN and a finite y, ct
+          // stay as they are (a lost particle's are zero). The operations
+          // and their rounding are drift6's in this build.
+          const double pn = fast_rcp(1.0 + r[q].dp);
+          const double nl = __builtin_isnan(r[q].x) ? 0.0 : len * pn;
+          r[q].x = __builtin_fma(nl, r[q].px, r[q].x);
+          r[q].y = __builtin_fma(nl, r[q].py, r[q].y);
+          const double p2 = __builtin_fma(r[q].px, r[q].px, r[q].py * r[q].py);
+          const double dct = (pn * nl) * p2;
+          const double hct = dct * 0.5;
+          r[q].ct = r[q].ct + hct;
+#endif
+        }
       } else if (h.kind == K_IDENTITY && h.flags == 0) {
         // A marker or monitor without misalignment or apertures
         // (IdentityPass.c:20-43) changes no coordinate, so its loss check
